@@ -1,0 +1,116 @@
+/* include/g2host.h -- host-side C ABI of the MI355X subgraph-search library.
+ *
+ * libg2host.so holds everything that runs on the CPU around the device path:
+ * CSR graph files, the synthetic workload generators, the degeneracy recode
+ * and view-bin maps, query patterns with their symmetry-breaking conditions,
+ * and the query plan.  No HIP types cross this boundary; all sizes are u64,
+ * vertex ids u32, row offsets u64 (the reference's uintV/uintE,
+ * common/meta.h:15-16).
+ *
+ * Return convention: 0 = OK, < 0 = error; g2h_last_error() describes the last
+ * error of the calling thread.
+ */
+#ifndef G2HOST_H
+#define G2HOST_H
+
+#include <stdint.h>
+#include "g2plan.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- CSR graph handle (replaces common/graph.h:57-107 `Graph` data side) ---- */
+typedef struct g2h_graph g2h_graph;
+
+/* .bin reader / writer, byte layout of common/graph.h:360-398 and :539-562:
+ * u64 sizeof(uintV)=4, u64 sizeof(uintE)=8, u64 V, u64 E, u64 row_ptrs[V+1],
+ * u32 cols[E]; no trailer. */
+int  g2h_graph_read_bin(const char* path, g2h_graph** out);
+int  g2h_graph_write_bin(const g2h_graph* g, const char* path);
+/* wrap a copy of caller arrays (rows must be sorted) */
+int  g2h_graph_from_csr(const uint64_t* row_ptrs, uint64_t nv, const uint32_t* cols, uint64_t ne,
+                        g2h_graph** out);
+/* symmetrise + sort + drop self loops / parallel edges from an edge list */
+int  g2h_graph_from_edges(const uint32_t* src, const uint32_t* dst, uint64_t m, uint64_t nv,
+                          g2h_graph** out);
+uint64_t        g2h_graph_nv(const g2h_graph* g);
+uint64_t        g2h_graph_ne(const g2h_graph* g);
+const uint64_t* g2h_graph_row_ptrs(const g2h_graph* g);
+const uint32_t* g2h_graph_cols(const g2h_graph* g);
+uint64_t        g2h_graph_max_degree(const g2h_graph* g);
+void            g2h_graph_free(g2h_graph* g);
+
+/* Graph::Preprocess (common/graph.h:720-811): drop self loops, parallel
+ * edges and degree-0 vertices, relabel densely in id order.  In place. */
+int  g2h_graph_preprocess(g2h_graph* g);
+
+/* kcore recode (kmeans/kcore.cpp:39-191): level-synchronous peeling, each
+ * k-shell sorted by degree, vertex ids recoded into that order.  In place.
+ * order_out (nullable, nv entries) receives old id -> new id.
+ * shell_offsets_out (nullable, nv+2 entries) receives the shell boundaries in
+ * the new order; *num_shells their count. */
+int  g2h_graph_kcore_recode(g2h_graph* g, uint32_t* order_out, uint64_t* shell_offsets_out,
+                            uint64_t* num_shells, uint32_t* degeneracy);
+
+/* view-bin map of kmeans/kcore.cpp:193-229 ("kcore1" blocks of
+ * floor(w/nb)+1 per shell) for a graph already recoded; bins[v] in [0,nb) */
+int  g2h_kcore_bins(const uint64_t* shell_offsets, uint64_t num_shells, uint32_t nb,
+                    uint32_t* bins, uint64_t nv);
+/* .vbmap text file (view/view_bin_manager.h:42-46): "K b0 b1 ... b(V-1)" */
+int  g2h_write_vbmap(const char* path, uint32_t nbins, const uint32_t* bins, uint64_t nv);
+int  g2h_read_vbmap(const char* path, uint32_t* nbins, uint32_t* bins, uint64_t nv);
+
+/* ---- synthetic workloads (seeded: splitmix64 -> xoshiro256**) ---- */
+/* Chung-Lu: w_i = w_max*(1+i/i0)^(-1/(gamma-1)) with i0 solved so that
+ * sum(w) = 2*target_edges, clipped below at w_min; round(sum/2) endpoint
+ * pairs drawn proportional to w; self loops and duplicates removed. */
+int  g2h_gen_chung_lu(uint64_t n, double gamma, double w_min, double w_max, uint64_t target_edges,
+                      uint64_t seed, g2h_graph** out);
+/* R-MAT: 2^scale vertices, edge_factor*2^scale draws with quadrant
+ * probabilities (a,b,c,1-a-b-c), no noise, then a seeded vertex permutation;
+ * symmetrised, deduplicated, no self loops (isolated vertices kept). */
+int  g2h_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                  uint64_t seed, g2h_graph** out);
+/* Chung-Lu backbone plus planted overlapping communities: ncomm communities
+ * with sizes drawn from a discrete power law P(s) ~ s^-comm_alpha on
+ * [smin,smax], members uniform over the vertices, intra-edge probability
+ * p ~ U[pmin,pmax] resampled (at most 64 times, else pmin) until the expected
+ * number of maximal cliques of G(s,p) is <= ecap. */
+int  g2h_gen_social(uint64_t n, double gamma, double w_min, double w_max, uint64_t backbone_edges,
+                    uint64_t ncomm, uint32_t smin, uint32_t smax, double comm_alpha,
+                    double pmin, double pmax, double ecap, uint64_t seed, g2h_graph** out);
+
+/* ---- query patterns and plans ---- */
+/* preset pattern (common/graph.h:813-1104), remapped as Graph::ReMapVertexId
+ * (common/graph.h:171-266); adjacency returned as bitmasks qadj[u] */
+int  g2h_query_preset(int preset, uint32_t* n, uint32_t qadj[G2_MAXQ]);
+/* remap an arbitrary connected pattern given by adjacency bitmasks */
+int  g2h_query_remap(uint32_t n, const uint32_t qadj_in[G2_MAXQ], uint32_t qadj_out[G2_MAXQ]);
+/* symmetry-breaking pairs (a,b) meaning M(a) < M(b): the reference's
+ * Graph::GetConditions (common/graph.h:564-686) over the full automorphism
+ * group, enumerated directly instead of through bliss. pairs[2*k], pairs[2*k+1] */
+int  g2h_query_conditions(uint32_t n, const uint32_t qadj[G2_MAXQ], uint32_t* npairs,
+                          uint32_t pairs[2 * G2_MAXQ * G2_MAXQ]);
+uint64_t g2h_query_automorphisms(uint32_t n, const uint32_t qadj[G2_MAXQ]);
+/* full plan of system/plan.h:51-372 for a remapped pattern */
+int  g2h_plan_build(uint32_t n, const uint32_t qadj[G2_MAXQ], g2_plan* plan);
+
+/* seeded random connected pattern: n vertices, density in [0,1]: a random
+ * spanning tree plus random extra edges up to round(density*n(n-1)/2) */
+int  g2h_query_random(uint32_t n, double density, uint64_t seed, uint32_t qadj[G2_MAXQ]);
+
+/* ---- view bins (view/view_bin.h:41-159) ---- */
+/* hop closure of the sources: vertices within hop-1 steps of a source whose
+ * degree is >= root_degree keep their full rows; every other row is empty.
+ * Output arrays: row_ptrs (nv+1) and cols (capacity cols_cap). */
+int  g2h_view_materialize(const g2h_graph* g, const uint32_t* sources, uint64_t ns, int hop,
+                          uint64_t root_degree, uint64_t* row_ptrs_out, uint32_t* cols_out,
+                          uint64_t cols_cap, uint64_t* ne_out);
+
+const char* g2h_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
