@@ -1,0 +1,182 @@
+// csrc/g2_ctx.hip -- device context, memory budget and the C ABI entry points.
+//
+// Replaces the reference executor's setup (system/pipeline_executor.h:44-99,
+// system/appbase.h:93-114), which grabs all free device memory for its level
+// buffers: here every allocation is charged to an explicit per-context budget
+// so that several consumers can share one MI355X, and buffers grow on demand.
+#include <algorithm>
+#include <cstring>
+
+#include "g2_internal.hpp"
+
+namespace g2 {
+
+int set_err(g2_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_err(g2_ctx* c, hipError_t e, const char* what) {
+    return set_err(c, G2_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure(g2_ctx* c, DevBuf& b, uint64_t bytes, const char* what) {
+    if (bytes <= b.bytes) return G2_OK;
+    bytes = (bytes + 255) & ~255ull;
+    const uint64_t after = c->used - b.bytes + bytes;
+    if (c->budget && after > c->budget)
+        return set_err(c, G2_EBUDGET, std::string(what) + ": needs " + std::to_string(bytes) +
+                                          " B, budget left " + std::to_string(c->budget - (c->used - b.bytes)));
+    if (b.ptr) {
+        hipError_t e = hipFree(b.ptr);
+        if (e != hipSuccess) return hip_err(c, e, "hipFree");
+        c->used -= b.bytes;
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) return hip_err(c, e, what);
+    b.bytes = bytes;
+    c->used += bytes;
+    return G2_OK;
+}
+
+static void release(g2_ctx* c, DevBuf& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    c->used -= b.bytes;
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+}  // namespace g2
+
+using namespace g2;
+
+extern "C" {
+
+int g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* name, uint32_t name_len) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return G2_EHIP;
+    if (num_cus) *num_cus = p.multiProcessorCount;
+    if (total_mem) *total_mem = p.totalGlobalMem;
+    if (name && name_len) {
+        std::strncpy(name, p.gcnArchName, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return G2_OK;
+}
+
+int g2_create(g2_ctx** out, int device, uint64_t budget, const g2_tuning* t) {
+    if (!out) return G2_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return G2_EHIP;
+    if (device < 0 || device >= ndev) return G2_EINVAL;
+    auto* c = new g2_ctx;
+    c->device = device;
+    c->budget = budget;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) { delete c; return G2_EHIP; }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return G2_EHIP; }
+    c->num_cus = p.multiProcessorCount;
+    if (t) c->tune = *t;
+    if (!c->tune.waves_per_cu) c->tune.waves_per_cu = 8;
+    if (!c->tune.block_waves) c->tune.block_waves = 4;
+    if (!c->tune.wave_scratch_words) c->tune.wave_scratch_words = 1ull << 16;
+    if (!c->tune.big_waves) c->tune.big_waves = 256;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return G2_EHIP; }
+    for (auto& ev : c->ev)
+        if (hipEventCreate(&ev) != hipSuccess) { g2_destroy(c); return G2_EHIP; }
+    *out = c;
+    return G2_OK;
+}
+
+void g2_destroy(g2_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
+                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order})
+        release(c, *b);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* g2_last_error(const g2_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* cols, uint64_t ne,
+                   const uint32_t* src, uint64_t ns) {
+    if (!c || !rp || (ne && !cols) || (ns && !src)) return set_err(c, G2_EINVAL, "null argument");
+    if (rp[nv] != ne) return set_err(c, G2_EINVAL, "row_ptrs[nv] != ne");
+    if (nv >= 0xffffffffull) return set_err(c, G2_EINVAL, "vertex ids must fit in u32");
+    for (uint64_t i = 0; i < ns; i++)
+        if (src[i] >= nv) return set_err(c, G2_EINVAL, "source id out of range");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    int rc;
+    if ((rc = ensure(c, c->rp, (nv + 1) * 8, "row_ptrs"))) return rc;
+    if ((rc = ensure(c, c->cols, std::max<uint64_t>(ne, 1) * 4, "cols"))) return rc;
+    if ((rc = ensure(c, c->src, std::max<uint64_t>(ns, 1) * 4, "sources"))) return rc;
+    if ((e = hipMemcpyAsync(c->rp.ptr, rp, (nv + 1) * 8, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D row_ptrs");
+    if (ne && (e = hipMemcpyAsync(c->cols.ptr, cols, ne * 4, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D cols");
+    if (ns && (e = hipMemcpyAsync(c->src.ptr, src, ns * 4, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D sources");
+    c->nv = nv;
+    c->ne = ne;
+    c->ns = ns;
+    uint64_t md = 0;
+    for (uint64_t v = 0; v < nv; v++) md = std::max(md, rp[v + 1] - rp[v]);
+    c->max_degree = md;
+    // level-1 windows of the matching tasks: source i covers windows
+    // [off[i], off[i+1]) of gm_chunk row entries each (one window when 0)
+    c->gm_task_chunk = c->tune.gm_chunk;
+    c->h_gm_task_off.assign(ns + 1, 0);
+    for (uint64_t i = 0; i < ns; i++) {
+        const uint64_t d = rp[src[i] + 1] - rp[src[i]];
+        uint64_t w = 1;
+        if (c->gm_task_chunk && d > c->gm_task_chunk) w = (d + c->gm_task_chunk - 1) / c->gm_task_chunk;
+        c->h_gm_task_off[i + 1] = c->h_gm_task_off[i] + w;
+    }
+    if ((rc = ensure(c, c->gm_task_off, (ns + 1) * 8, "task offsets"))) return rc;
+    if ((e = hipMemcpyAsync(c->gm_task_off.ptr, c->h_gm_task_off.data(), (ns + 1) * 8, hipMemcpyHostToDevice, c->stream)))
+        return hip_err(c, e, "H2D task offsets");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "upload sync");
+    return G2_OK;
+}
+
+int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
+               uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
+    if (!c) return G2_EINVAL;
+    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (hist && hist_len < 2) return set_err(c, G2_EINVAL, "hist_len must be >= 2");
+    if (sched && sched->order) {
+        if (sched->n != c->ns) return set_err(c, G2_EINVAL, "schedule length != source count");
+        std::vector<uint8_t> seen(c->ns, 0);
+        for (uint64_t i = 0; i < c->ns; i++) {
+            const uint32_t o = sched->order[i];
+            if (o >= c->ns || seen[o]) return set_err(c, G2_EINVAL, "schedule is not a permutation of the sources");
+            seen[o] = 1;
+        }
+    }
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    return mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);
+}
+
+int g2_gm_set_plan(g2_ctx* c, const g2_plan* p) {
+    if (!c || !p) return G2_EINVAL;
+    return gm_compile(c, p);
+}
+
+int g2_gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_count, g2_stats* st) {
+    if (!c) return G2_EINVAL;
+    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (!c->has_plan) return set_err(c, G2_ENOPLAN, "no plan set");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    return gm_run(c, flags, count, per_root_count, st);
+}
+
+}  // extern "C"

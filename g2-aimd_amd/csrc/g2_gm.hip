@@ -1,0 +1,441 @@
+// csrc/g2_gm.hip -- plan-driven subgraph matching on gfx950.
+//
+// Counts the same matches as the reference's GMatch app
+// (app_gmatch/gmatch.h:83-674): a partial match of length id is extended by
+// position id of the plan's match order with
+//   candidates = row of the back neighbour with the smallest row (first on
+//                ties, gmatch.h:517-536), filtered by the conditions against
+//                earlier positions (gmatch.h:559-588) and by membership in the
+//                rows of the other back neighbours (gmatch.h:589-601);
+//   shared positions take a pre-intersection over the pre back neighbours,
+//                computed once per prefix and filtered per candidate of the
+//                previous position by the after conditions / rows
+//                (gmatch.h:273-509);
+//   the last position is counted, not stored (gmatch.h:604-608).
+// Counts are the overflow-free u64 totals.
+//
+// MI355X design: one wave64 walks one root (or one window of the root's row
+// at position 1) depth first.  Each level keeps its candidate list in the
+// wave's HBM arena (written by ballot compaction, read back as broadcasts);
+// order conditions become a [lo,hi) value range so only the qualifying slice
+// of every sorted row is streamed or searched.  The last level counts with a
+// per-lane u64 and the root total is one wave reduction.
+#include <algorithm>
+
+#include "g2_device.hpp"
+#include "g2_internal.hpp"
+
+namespace g2 {
+
+enum { GCTR_NEXT = 0, GCTR_COUNT, GCTR_BYTES, GCTR_NODES, GCTR_N };
+
+struct GmArgs {
+    const uint64_t* __restrict__ rp;
+    const uint32_t* __restrict__ cols;
+    const uint32_t* __restrict__ src;
+    const uint64_t* __restrict__ task_off;  // ns+1 window offsets per source
+    uint64_t ns;
+    uint64_t ntasks;
+    uint32_t chunk;                         // level-1 window length (0 = whole row)
+    GmProgram prog;
+    uint32_t* scratch;
+    uint64_t cap;                           // entries per list
+    unsigned long long* ctr;
+    unsigned long long* per_root;           // nullable
+};
+
+struct Range {
+    uint32_t lo;   // smallest admissible value
+    uint32_t hi;   // one past the largest admissible value (0xffffffff = none)
+};
+
+constexpr int kMaxBlockWaves = 4;
+
+struct WaveState {
+    uint32_t M[G2_MAXQ];
+    uint64_t len[G2_MAXQ];
+    uint64_t idx[G2_MAXQ];
+    uint64_t prelen[G2_MAXQ];
+};
+
+__device__ inline Range cond_range(const WaveState& S, uint32_t n, const uint32_t* op, const uint32_t* pos) {
+    Range r{0u, 0xffffffffu};
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t m = S.M[pos[k]];
+        if (op[k] == 1) r.lo = max(r.lo, m + 1);          // M(u) > M(j)
+        else if (op[k] == 0) r.hi = min(r.hi, m);          // M(u) < M(j)
+    }
+    return r;
+}
+
+__device__ inline bool neq_ok(const WaveState& S, uint32_t n, const uint32_t* op, const uint32_t* pos, uint32_t w) {
+    for (uint32_t k = 0; k < n; k++)
+        if (op[k] == 2 && S.M[pos[k]] == w) return false;
+    return true;
+}
+
+__device__ inline uint64_t store_bytes(const GmStep& st, uint32_t id, uint32_t n, uint64_t nout) {
+    if (id + 1 == n) return 0;
+    if (st.store_prefix) return 4 * (id + nout);
+    return 4 * (uint64_t)(id + 1) * nout;
+}
+
+// Filter the row of `parent` (restricted to [row_lo,row_hi) entries) by the
+// value range, NON_EQUAL conditions and membership in the other rows.  With
+// out != nullptr the survivors are appended (sorted) and their number is
+// returned; otherwise they are only counted into *lane_count (per lane).
+template <bool ACCOUNT>
+__device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn, const uint32_t* bn,
+                               uint32_t parent, uint64_t row_lo, uint64_t row_hi, Range rg,
+                               uint32_t ncond, const uint32_t* op, const uint32_t* pos, uint32_t* out,
+                               uint64_t* lane_count, uint64_t* bytes, uint64_t* npass_out) {
+    const uint32_t lane = lane_id();
+    const uint32_t* __restrict__ cols = a.cols;
+    uint64_t s0 = lower_bound(cols, row_lo, row_hi, rg.lo);
+    uint64_t s1 = rg.hi == 0xffffffffu ? row_hi : lower_bound(cols, s0, row_hi, rg.hi);
+    // narrowed rows of the other back neighbours
+    uint64_t olo[G2_MAXQ], ohi[G2_MAXQ];
+    uint32_t no = 0;
+    uint64_t bcost = 0;
+    for (uint32_t k = 0; k < nbn; k++) {
+        if (bn[k] == parent) continue;
+        const uint32_t m = S.M[bn[k]];
+        uint64_t l = a.rp[m], h = a.rp[m + 1];
+        if (ACCOUNT) bcost += cost_isect(1, h - l);
+        l = lower_bound(cols, l, h, rg.lo);
+        if (rg.hi != 0xffffffffu) h = lower_bound(cols, l, h, rg.hi);
+        olo[no] = l;
+        ohi[no] = h;
+        no++;
+    }
+    uint64_t nout = 0, npass = 0;
+    for (uint64_t base = s0; base < s1; base += kWave) {
+        const uint64_t e = base + lane;
+        bool ok = false, pass = false;
+        uint32_t w = 0;
+        if (e < s1) {
+            w = cols[e];
+            pass = neq_ok(S, ncond, op, pos, w);
+            ok = pass;
+            for (uint32_t k = 0; k < no && ok; k++) ok = contains(cols, olo[k], ohi[k], w);
+        }
+        if (ACCOUNT) npass += pass;
+        const uint64_t m = ballot(ok);
+        if (out) {
+            if (ok) out[nout + lanes_below(m)] = w;
+            nout += __popcll(m);
+        } else if (ok) {
+            (*lane_count)++;
+        }
+    }
+    if (ACCOUNT) {
+        npass = wave_sum(npass);
+        if (lane == 0) *bytes += npass * bcost;
+        if (npass_out) *npass_out = npass;
+    }
+    return nout;
+}
+
+// count or collect the after-filtered entries of a pre list for position id
+template <bool ACCOUNT>
+__device__ uint64_t filter_pre(const GmArgs& a, const WaveState& S, const GmStep& st, const uint32_t* pre,
+                               uint64_t plen, uint32_t* out, uint64_t* lane_count, uint64_t* bytes) {
+    const uint32_t lane = lane_id();
+    const Range rg = cond_range(S, st.nacond, st.acop, st.acpos);
+    // pre is sorted: restrict to the value range
+    const uint64_t p0 = lower_bound(pre, 0, plen, rg.lo);
+    const uint64_t p1 = rg.hi == 0xffffffffu ? plen : lower_bound(pre, p0, plen, rg.hi);
+    uint64_t alo[G2_MAXQ], ahi[G2_MAXQ];
+    for (uint32_t k = 0; k < st.nafter; k++) {
+        const uint32_t m = S.M[st.after[k]];
+        alo[k] = a.rp[m];
+        ahi[k] = a.rp[m + 1];
+        if (ACCOUNT && lane == 0) *bytes += cost_isect(plen, ahi[k] - alo[k]);
+        alo[k] = lower_bound(a.cols, alo[k], ahi[k], rg.lo);
+        if (rg.hi != 0xffffffffu) ahi[k] = lower_bound(a.cols, alo[k], ahi[k], rg.hi);
+    }
+    uint64_t nout = 0;
+    for (uint64_t base = p0; base < p1; base += kWave) {
+        const uint64_t e = base + lane;
+        bool ok = false;
+        uint32_t w = 0;
+        if (e < p1) {
+            w = pre[e];
+            ok = neq_ok(S, st.nacond, st.acop, st.acpos, w);
+            for (uint32_t k = 0; k < st.nafter && ok; k++) ok = contains(a.cols, alo[k], ahi[k], w);
+        }
+        const uint64_t m = ballot(ok);
+        if (out) {
+            if (ok) out[nout + lanes_below(m)] = w;
+            nout += __popcll(m);
+        } else if (ok) {
+            (*lane_count)++;
+        }
+    }
+    return nout;
+}
+
+// parent = back neighbour whose matched row is smallest, first on ties
+__device__ inline uint32_t min_row(const GmArgs& a, const WaveState& S, uint32_t nbn, const uint32_t* bn) {
+    uint32_t best = bn[0];
+    uint64_t bd = a.rp[S.M[bn[0]] + 1] - a.rp[S.M[bn[0]]];
+    for (uint32_t k = 1; k < nbn; k++) {
+        const uint32_t m = S.M[bn[k]];
+        const uint64_t d = a.rp[m + 1] - a.rp[m];
+        if (d < bd) { bd = d; best = bn[k]; }
+    }
+    return best;
+}
+
+template <bool ACCOUNT>
+__global__ __launch_bounds__(256) void gm_kernel(GmArgs a) {
+    __shared__ WaveState states[kMaxBlockWaves];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x / kWave;
+    WaveState& S = states[wib];
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wib;
+    const uint32_t n = a.prog.n;
+    const uint64_t cap = a.cap;
+    uint32_t* ws = a.scratch + gwave * cap * (2 * G2_MAXQ);
+    auto L = [&](uint32_t id) { return ws + (uint64_t)id * cap; };
+    auto PRE = [&](uint32_t id) { return ws + (uint64_t)(G2_MAXQ + id) * cap; };
+    uint64_t lane_total = 0, bytes = 0, nodes = 0;
+
+    while (true) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(&a.ctr[GCTR_NEXT], 1ull);
+        t = uniform((uint64_t)t);
+        if (t >= a.ntasks) break;
+        // task -> (source, level-1 window)
+        uint64_t lo = 0, hi = a.ns;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (a.task_off[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint64_t r = uniform(lo);
+        const uint64_t win = t - a.task_off[r];
+        const uint32_t v0 = uniform(a.src[r]);
+        const uint64_t rlo = a.rp[v0], rhi = a.rp[v0 + 1];
+        uint64_t wlo = rlo, whi = rhi;
+        if (a.chunk) {
+            wlo = min(rhi, rlo + win * a.chunk);
+            whi = min(rhi, wlo + a.chunk);
+        }
+        if (lane == 0) S.M[0] = v0;
+        wave_sync();
+        const uint64_t before = lane_total;
+        if (ACCOUNT && win == 0 && lane == 0) bytes += 4 + 4 * 1 + 4 * (rhi - rlo);
+
+        // position 1: the root's row (window)
+        {
+            const GmStep& st = a.prog.step[1];
+            const Range rg = cond_range(S, st.ncond, st.cop, st.cpos);
+            uint64_t nout = 0, bsum = 0, npass = 0;
+            nodes++;
+            if (n == 2) {
+                filter_row<ACCOUNT>(a, S, st.nbn, st.bn, 0, wlo, whi, rg, st.ncond, st.cop, st.cpos, nullptr,
+                                    &lane_total, &bsum, &npass);
+            } else {
+                nout = filter_row<ACCOUNT>(a, S, st.nbn, st.bn, 0, wlo, whi, rg, st.ncond, st.cop, st.cpos, L(1),
+                                           nullptr, &bsum, &npass);
+                if (ACCOUNT && lane == 0) {
+                    bsum += store_bytes(st, 1, n, nout);
+                    // a PREFIX store writes the prefix once per root, not per window
+                    if (win && st.store_prefix) bsum -= 4;
+                }
+                if (lane == 0) { S.len[1] = nout; S.idx[1] = 0; }
+            }
+            if (ACCOUNT && lane == 0) bytes += bsum;
+        }
+        wave_sync();
+        if (n > 2) {
+            // pre-intersection of position 2 if shared (depends on position 0 only)
+            uint32_t depth = 1;
+            auto build_pre = [&](uint32_t id, bool charge) {
+                // prefix = positions < id-1 are matched
+                const GmStep& st = a.prog.step[id];
+                const uint32_t parent = min_row(a, S, st.npre, st.pre);
+                const uint32_t pm = S.M[parent];
+                const Range rg = cond_range(S, st.npcond, st.pcop, st.pcpos);
+                uint64_t bsum = 0;
+                const uint64_t len = filter_row<ACCOUNT>(a, S, st.npre, st.pre, parent, a.rp[pm], a.rp[pm + 1], rg,
+                                                         st.npcond, st.pcop, st.pcpos, PRE(id), nullptr, &bsum, nullptr);
+                if (lane == 0) {
+                    S.prelen[id] = len;
+                    if (ACCOUNT && charge) bytes += bsum + 4 * (id - 1) + 4 * (a.rp[pm + 1] - a.rp[pm]);
+                }
+                wave_sync();
+            };
+            if (a.prog.step[2].shared) build_pre(2, win == 0);  // same list for every window
+            while (true) {
+                const uint32_t id = depth;
+                const uint64_t ix = S.idx[id];
+                if (ix >= S.len[id]) {
+                    if (depth == 1) break;
+                    depth--;
+                    continue;
+                }
+                const uint32_t cand = L(id)[ix];
+                if (lane == 0) { S.M[id] = cand; S.idx[id] = ix + 1; }
+                wave_sync();
+                const uint32_t nx = id + 1;
+                const GmStep& st = a.prog.step[nx];
+                nodes++;
+                uint64_t bsum = 0;
+                if (st.shared) {
+                    if (nx + 1 == n) {
+                        filter_pre<ACCOUNT>(a, S, st, PRE(nx), S.prelen[nx], nullptr, &lane_total, &bsum);
+                    } else {
+                        const uint64_t nout = filter_pre<ACCOUNT>(a, S, st, PRE(nx), S.prelen[nx], L(nx), nullptr, &bsum);
+                        if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
+                        if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
+                    }
+                } else {
+                    const uint32_t parent = min_row(a, S, st.nbn, st.bn);
+                    const uint32_t pm = S.M[parent];
+                    const Range rg = cond_range(S, st.ncond, st.cop, st.cpos);
+                    if (ACCOUNT && lane == 0) bsum += 4 * nx + 4 * (a.rp[pm + 1] - a.rp[pm]);
+                    if (nx + 1 == n) {
+                        filter_row<ACCOUNT>(a, S, st.nbn, st.bn, parent, a.rp[pm], a.rp[pm + 1], rg, st.ncond, st.cop,
+                                            st.cpos, nullptr, &lane_total, &bsum, nullptr);
+                    } else {
+                        const uint64_t nout = filter_row<ACCOUNT>(a, S, st.nbn, st.bn, parent, a.rp[pm], a.rp[pm + 1],
+                                                                  rg, st.ncond, st.cop, st.cpos, L(nx), nullptr, &bsum,
+                                                                  nullptr);
+                        if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
+                        if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
+                    }
+                }
+                if (ACCOUNT && lane == 0) bytes += bsum;
+                wave_sync();
+                if (nx + 1 < n) {
+                    if (a.prog.step[nx + 1].shared) build_pre(nx + 1, true);
+                    depth = nx;
+                }
+            }
+        }
+        if (a.per_root) {
+            const uint64_t mine = wave_sum(lane_total - before);
+            if (lane == 0 && mine) atomicAdd(&a.per_root[r], (unsigned long long)mine);
+        }
+    }
+    const uint64_t tot = wave_sum(lane_total);
+    if (ACCOUNT) bytes = wave_sum(bytes);
+    if (lane == 0) {
+        if (tot) atomicAdd(&a.ctr[GCTR_COUNT], (unsigned long long)tot);
+        if (ACCOUNT && bytes) atomicAdd(&a.ctr[GCTR_BYTES], (unsigned long long)bytes);
+        atomicAdd(&a.ctr[GCTR_NODES], (unsigned long long)nodes);
+    }
+}
+
+int gm_compile(g2_ctx* c, const g2_plan* p) {
+    if (p->n < 2 || p->n > G2_MAXQ) return set_err(c, G2_EINVAL, "plan size must be 2..8");
+    const uint32_t n = p->n;
+    GmProgram g{};
+    g.n = n;
+    for (uint32_t id = 1; id < n; id++) {
+        GmStep& s = g.step[id];
+        const uint32_t u = p->seq[id];
+        if (p->bn_count[id] == 0 || p->bn_count[id] > G2_MAXQ) return set_err(c, G2_EINVAL, "position without back neighbour");
+        s.nbn = p->bn_count[id];
+        for (uint32_t k = 0; k < s.nbn; k++) {
+            const uint32_t b = p->bn[id][k];
+            if (b >= n || p->rev[b] >= id) return set_err(c, G2_EINVAL, "back neighbour not matched earlier");
+            s.bn[k] = p->rev[b];
+        }
+        s.ncond = p->cond_count[u];
+        for (uint32_t k = 0; k < s.ncond; k++) {
+            s.cop[k] = p->cond[u][k][0];
+            s.cpos[k] = p->rev[p->cond[u][k][1]];
+            if (s.cop[k] > 2 || s.cpos[k] >= id) return set_err(c, G2_EINVAL, "bad condition");
+        }
+        s.shared = p->shared[id] ? 1 : 0;
+        if (s.shared) {
+            if (id < 2 || p->pre_bn_count[id] == 0) return set_err(c, G2_EINVAL, "shared position without pre back neighbours");
+            s.npre = p->pre_bn_count[id];
+            for (uint32_t k = 0; k < s.npre; k++) {
+                s.pre[k] = p->rev[p->pre_bn[id][k]];
+                if (s.pre[k] + 1 >= id) return set_err(c, G2_EINVAL, "pre back neighbour not in the prefix");
+            }
+            s.nafter = p->after_bn_count[id];
+            for (uint32_t k = 0; k < s.nafter; k++) {
+                s.after[k] = p->rev[p->after_bn[id][k]];
+                if (s.after[k] + 1 != id) return set_err(c, G2_EINVAL, "after back neighbour is not the previous position");
+            }
+            s.npcond = p->pre_cond_count[u];
+            for (uint32_t k = 0; k < s.npcond; k++) {
+                s.pcop[k] = p->pre_cond[u][k][0];
+                s.pcpos[k] = p->rev[p->pre_cond[u][k][1]];
+                if (s.pcop[k] > 2 || s.pcpos[k] + 1 >= id) return set_err(c, G2_EINVAL, "bad pre condition");
+            }
+            s.nacond = p->after_cond_count[u];
+            for (uint32_t k = 0; k < s.nacond; k++) {
+                s.acop[k] = p->after_cond[u][k][0];
+                s.acpos[k] = p->rev[p->after_cond[u][k][1]];
+                if (s.acop[k] > 2 || s.acpos[k] + 1 != id) return set_err(c, G2_EINVAL, "bad after condition");
+            }
+        }
+        s.store_prefix = (id + 1 < n && p->strategy[id + 1] == 1) ? 1 : 0;
+    }
+    if (g.step[1].shared) return set_err(c, G2_EINVAL, "position 1 cannot be shared");
+    c->prog = g;
+    c->plan = *p;
+    c->has_plan = true;
+    return G2_OK;
+}
+
+int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_stats* st) {
+    const bool account = flags & G2_RUN_ACCOUNT;
+    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
+    uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
+    const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
+    const uint64_t per_wave = cap * 2 * G2_MAXQ * 4;
+    // shrink the grid until the arenas fit the budget
+    while (nblocks > 1 && c->budget && c->used - c->scratch.bytes + nblocks * bw * per_wave > c->budget) nblocks /= 2;
+    int rc;
+    if ((rc = ensure(c, c->counters, GCTR_N * 8, "counters"))) return rc;
+    if ((rc = ensure(c, c->scratch, nblocks * bw * per_wave, "wave arenas"))) return rc;
+    if (per_root && (rc = ensure(c, c->per_root, std::max<uint64_t>(c->ns, 1) * 8, "per-root counts"))) return rc;
+    hipError_t e;
+    auto* ctr = (unsigned long long*)c->counters.ptr;
+    if ((e = hipMemsetAsync(ctr, 0, GCTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
+    if (per_root && (e = hipMemsetAsync(c->per_root.ptr, 0, c->ns * 8, c->stream))) return hip_err(c, e, "memset per-root");
+    GmArgs a{};
+    a.rp = (const uint64_t*)c->rp.ptr;
+    a.cols = (const uint32_t*)c->cols.ptr;
+    a.src = (const uint32_t*)c->src.ptr;
+    a.task_off = (const uint64_t*)c->gm_task_off.ptr;
+    a.ns = c->ns;
+    a.ntasks = c->h_gm_task_off.empty() ? 0 : c->h_gm_task_off.back();
+    a.chunk = c->gm_task_chunk;
+    a.prog = c->prog;
+    a.scratch = (uint32_t*)c->scratch.ptr;
+    a.cap = cap;
+    a.ctr = ctr;
+    a.per_root = per_root ? (unsigned long long*)c->per_root.ptr : nullptr;
+    if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
+    if (account) hipLaunchKernelGGL(gm_kernel<true>, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a);
+    else hipLaunchKernelGGL(gm_kernel<false>, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a);
+    if ((e = hipGetLastError())) return hip_err(c, e, "gm launch");
+    if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
+    unsigned long long h[GCTR_N];
+    if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
+    if (per_root && c->ns && (e = hipMemcpyAsync(per_root, c->per_root.ptr, c->ns * 8, hipMemcpyDeviceToHost, c->stream)))
+        return hip_err(c, e, "D2H per-root");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "gm sync");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    if (count) *count = h[GCTR_COUNT];
+    if (st) {
+        st->kernel_ms = ms;
+        st->launches = 1;
+        st->total_ms = ms;
+        st->nodes = h[GCTR_NODES];
+        st->alg_bytes = account ? h[GCTR_BYTES] : 0;
+        st->big_roots = 0;
+        st->max_scratch_words = cap;
+    }
+    return G2_OK;
+}
+
+}  // namespace g2

@@ -1,0 +1,87 @@
+// csrc/g2_internal.hpp -- device-runtime internals shared by the .hip units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "g2hip.h"
+
+namespace g2 {
+
+// device buffer that grows on demand and is charged to the context budget
+struct DevBuf {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+// matching plan as the kernels read it: every table indexed by POSITION in
+// the match order, back neighbours and condition operands as positions
+struct GmStep {
+    uint32_t nbn;                 // back neighbours of this position
+    uint32_t bn[G2_MAXQ];
+    uint32_t ncond;               // conditions against earlier positions
+    uint32_t cop[G2_MAXQ];        // 0: M(u) < M(j)   1: M(u) > M(j)   2: M(u) != M(j)
+    uint32_t cpos[G2_MAXQ];
+    uint32_t shared;              // candidates come from the pre-intersection
+    uint32_t npre, pre[G2_MAXQ];  // pre back neighbours (positions < id-1)
+    uint32_t npcond, pcop[G2_MAXQ], pcpos[G2_MAXQ];
+    uint32_t nafter, after[G2_MAXQ];
+    uint32_t nacond, acop[G2_MAXQ], acpos[G2_MAXQ];
+    uint32_t store_prefix;        // strategy[id+1] == PREFIX (byte model only)
+};
+
+struct GmProgram {
+    uint32_t n;
+    GmStep step[G2_MAXQ];
+};
+
+}  // namespace g2
+
+struct g2_ctx {
+    int device = 0;
+    uint32_t num_cus = 0;
+    uint64_t budget = 0;          // 0 = no limit beyond device memory
+    uint64_t used = 0;
+    g2_tuning tune{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+
+    // view
+    g2::DevBuf rp, cols, src;
+    uint64_t nv = 0, ne = 0, ns = 0;
+    uint64_t max_degree = 0;
+    std::vector<uint64_t> h_gm_task_off;  // host copy: level-1 window offsets per source
+    g2::DevBuf gm_task_off;
+    uint32_t gm_task_chunk = 0;
+
+    // run state
+    g2::DevBuf counters;          // small counters / flags
+    g2::DevBuf scratch;           // per-wave arenas
+    g2::DevBuf big_scratch;
+    g2::DevBuf list;              // deferred roots
+    g2::DevBuf hist;
+    g2::DevBuf per_root;
+    g2::DevBuf order;
+
+    // matching
+    bool has_plan = false;
+    g2::GmProgram prog{};
+    g2_plan plan{};
+};
+
+namespace g2 {
+
+int set_err(g2_ctx* c, int code, const std::string& msg);
+int hip_err(g2_ctx* c, hipError_t e, const char* what);
+// make sure buf holds at least `bytes`; charged against the budget
+int ensure(g2_ctx* c, DevBuf& buf, uint64_t bytes, const char* what);
+
+int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
+            uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st);
+int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_count, g2_stats* st);
+int gm_compile(g2_ctx* c, const g2_plan* p);
+
+}  // namespace g2

@@ -1,0 +1,421 @@
+// csrc/g2_mce.hip -- maximal clique enumeration on gfx950.
+//
+// Same search tree as the reference's BK app (app_BK/BK.h:78-474):
+//   root node of v:  R = {v}, P = {u in N(v): u > v}, X = {u in N(v): u < v}
+//                    (BK.h:323-355; degree-0 roots are skipped, :335)
+//   clique test:     P and X empty (BK.h:78-91)
+//   pivot:           argmax_{u in P u X} |P & N(u)|, ties to the LAST u in list
+//                    order, and list order is vertex-id order (BK.h:282-321)
+//   children:        q in P \ N(pivot) in increasing id; earlier q's move to X
+//                    (BK.h:445-474 with the Q relabelling of :236-280)
+// so totals, size histograms and per-root node counts are bit-exact with it.
+//
+// MI355X design (nothing of the reference's data layout survives):
+//   * one wave64 owns one root at a time (dynamic fetch from a global counter)
+//     and walks the root's subtree depth first with an explicit stack, instead
+//     of the reference's level-synchronous global queues;
+//   * the root's neighbourhood C = N(v) (sorted, position order == id order)
+//     becomes bitsets: adjP[k] = N(C[k]) & P0 over the p0 = |P0| later
+//     neighbours, and adjX[j] = N(P0[j]) & X0 over the s earlier ones.  Set
+//     intersection is then AND + popcount on 64-bit words;
+//   * a node is (P, XP) in P0 space (one 64-bit word per lane, lane i holds
+//     word i) plus XX in X0 space (a stack frame in the wave's HBM arena);
+//   * pivot scoring gives each candidate to one lane and reduces the key
+//     (score << 32 | position) with a 64-bit wave max: max score, then the
+//     largest position == the reference's "last maximum" rule.
+// Roots whose bitsets exceed the per-wave arena are deferred to a second pass
+// with a larger arena (never dropped).
+#include <algorithm>
+
+#include "g2_device.hpp"
+#include "g2_internal.hpp"
+
+namespace g2 {
+
+enum { CTR_NEXT = 0, CTR_CLIQUES, CTR_NODES, CTR_BYTES, CTR_NBIG, CTR_MAXNEED, CTR_ERR, CTR_N };
+
+struct MceArgs {
+    const uint64_t* __restrict__ rp;
+    const uint32_t* __restrict__ cols;
+    const uint32_t* __restrict__ src;
+    const uint32_t* __restrict__ order;  // task -> source index (nullable)
+    const uint32_t* __restrict__ list;   // task -> source index of the large-root pass (nullable)
+    uint64_t ntasks;
+    uint64_t* scratch;
+    uint64_t scratch_words;              // per wave
+    uint32_t* big_list;                  // deferred source indices (null in the large-root pass)
+    unsigned long long* ctr;
+    unsigned long long* hist;
+    uint32_t hist_len;
+    unsigned long long* per_root;        // nullable
+};
+
+// u64 words of the per-root arena: adjP (d*W) + adjX (p0*WX) + (p0+1) frames
+__host__ __device__ inline uint64_t mce_need(uint64_t d, uint64_t s) {
+    const uint64_t p0 = d - s, W = (p0 + 63) / 64, WX = (s + 63) / 64;
+    return d * W + p0 * WX + (p0 + 1) * (3 * W + WX);
+}
+
+__device__ inline uint64_t ones(uint64_t nbits) { return nbits >= 64 ? ~0ull : ((1ull << nbits) - 1); }
+
+// The arena is private to the wave, so workgroup-scope ordering is enough to
+// make the zero stores precede the atomics and the atomics precede the reads
+// (the CU's L1 is shared by the whole workgroup; agent scope would write back
+// the XCD's L2 on every root).
+__device__ inline void arena_sync() { wave_sync(); }
+
+__device__ inline uint64_t degree_of(const uint64_t* __restrict__ rp, uint32_t u) { return rp[u + 1] - rp[u]; }
+
+template <bool ACCOUNT>
+__global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
+    __shared__ unsigned long long lhist[64];
+    for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x) lhist[i] = 0;
+    __syncthreads();
+
+    const uint32_t lane = lane_id();
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    uint64_t* ws = a.scratch + gwave * a.scratch_words;
+    uint64_t my_cliques = 0, my_nodes = 0, my_bytes = 0;
+
+    while (true) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(&a.ctr[CTR_NEXT], 1ull);
+        t = uniform((uint64_t)t);
+        if (t >= a.ntasks) break;
+        const uint32_t si = uniform(a.list ? a.list[t] : (a.order ? a.order[t] : (uint32_t)t));
+        const uint32_t v = uniform(a.src[si]);
+        const uint64_t lo = uniform(a.rp[v]);
+        const uint64_t d = uniform(a.rp[v + 1]) - lo;
+        const uint32_t* __restrict__ C = a.cols + lo;
+        uint64_t root_nodes = 0;
+
+        if (d == 0) {  // no subgraph is generated for an isolated root
+            if (a.per_root && lane == 0) a.per_root[si] = 0;
+            continue;
+        }
+        const uint64_t s = uniform(lower_bound(C, 0, d, v));
+        const uint64_t p0 = d - s;
+        if (p0 == 0) {  // root node with empty P and non-empty X: one node, no clique
+            if (ACCOUNT && lane == 0) my_bytes += 5 * (1 + d);
+            my_nodes += 1;
+            if (a.per_root && lane == 0) a.per_root[si] = 1;
+            continue;
+        }
+        const uint64_t W = (p0 + 63) / 64, WX = (s + 63) / 64, FW = 3 * W + WX;
+        const uint64_t need = mce_need(d, s);
+        if (W > kWave) {  // P sets wider than 64 words per lane are not supported
+            if (lane == 0) atomicMax(&a.ctr[CTR_ERR], (unsigned long long)need);
+            continue;
+        }
+        if (need > a.scratch_words) {
+            if (lane == 0) {
+                if (a.big_list) {
+                    const unsigned long long idx = atomicAdd(&a.ctr[CTR_NBIG], 1ull);
+                    a.big_list[idx] = si;
+                    atomicMax(&a.ctr[CTR_MAXNEED], (unsigned long long)need);
+                } else {
+                    atomicMax(&a.ctr[CTR_ERR], (unsigned long long)need);
+                }
+            }
+            continue;
+        }
+        uint64_t* adjP = ws;                 // d rows of W words
+        uint64_t* adjX = ws + d * W;         // p0 rows of WX words
+        uint64_t* frames = adjX + p0 * WX;   // (p0+1) frames of FW words
+
+        // ---- build the neighbourhood bitsets ------------------------------
+        for (uint64_t i = lane; i < d * W + p0 * WX; i += kWave) ws[i] = 0;
+        arena_sync();
+        for (uint64_t j = 0; j < p0; j++) {
+            const uint32_t p = C[s + j];
+            const uint64_t plo = a.rp[p], dp = a.rp[p + 1] - plo;
+            const uint32_t* __restrict__ Np = a.cols + plo;
+            uint64_t* rowP = adjP + (s + j) * W;
+            uint64_t* rowX = adjX + j * WX;
+            const uint64_t jw = j >> 6, jb = 1ull << (j & 63);
+            if (dp <= d) {
+                // stream N(p) inside [C[0], C[d-1]] and locate each entry in C
+                const uint64_t e0 = lower_bound(Np, 0, dp, C[0]);
+                const uint64_t e1 = lower_bound(Np, e0, dp, C[d - 1] + 1);
+                for (uint64_t e = e0 + lane; e < e1; e += kWave) {
+                    const uint32_t w = Np[e];
+                    if (w < v) {
+                        const uint64_t k = lower_bound(C, 0, s, w);
+                        if (k < s && C[k] == w) {
+                            atomicOr((unsigned long long*)&rowX[k >> 6], 1ull << (k & 63));
+                            atomicOr((unsigned long long*)&adjP[k * W + jw], jb);
+                        }
+                    } else {
+                        const uint64_t k = lower_bound(C, s, d, w);
+                        if (k < d && C[k] == w) {
+                            const uint64_t kk = k - s;
+                            atomicOr((unsigned long long*)&rowP[kk >> 6], 1ull << (kk & 63));
+                        }
+                    }
+                }
+            } else {
+                // stream C and look each entry up in the longer N(p)
+                for (uint64_t base = 0; base < d; base += kWave) {
+                    const uint64_t k = base + lane;
+                    if (k < d && contains(Np, 0, dp, C[k])) {
+                        if (k < s) {
+                            atomicOr((unsigned long long*)&rowX[k >> 6], 1ull << (k & 63));
+                            atomicOr((unsigned long long*)&adjP[k * W + jw], jb);
+                        } else {
+                            const uint64_t kk = k - s;
+                            atomicOr((unsigned long long*)&rowP[kk >> 6], 1ull << (kk & 63));
+                        }
+                    }
+                }
+            }
+        }
+        arena_sync();
+
+        // ---- depth-first walk of the root's subtree -----------------------
+        uint64_t pw = 0, xpw = 0, qw = 0;
+        if (lane < W) pw = ones(p0 - (uint64_t)lane * 64);
+        for (uint64_t i = lane; i < WX; i += kWave) frames[3 * W + i] = ones(s - i * 64);
+        uint64_t depth = 0;
+        bool fresh = true;
+        while (true) {
+            uint64_t* F = frames + depth * FW;
+            uint64_t* XX = F + 3 * W;
+            bool descend = false;
+            if (fresh) {
+                root_nodes++;
+                const bool p_any = ballot(pw != 0) != 0;
+                uint64_t xx_or = 0;
+                for (uint64_t i = lane; i < WX; i += kWave) xx_or |= XX[i];
+                const bool x_any = (ballot(xpw != 0) | ballot(xx_or != 0)) != 0;
+                uint64_t np = 0, nPX = 0;
+                if (ACCOUNT) {
+                    uint64_t cx = 0;
+                    for (uint64_t i = lane; i < WX; i += kWave) cx += __popcll(XX[i]);
+                    np = wave_sum(__popcll(pw));
+                    nPX = np + wave_sum(__popcll(xpw)) + wave_sum(cx);
+                    if (lane == 0) my_bytes += 5 * (depth + 1 + nPX) * (depth ? 2 : 1);
+                }
+                if (!p_any) {
+                    if (!x_any && lane == 0) {  // maximal clique R, |R| = depth + 1
+                        my_cliques++;
+                        const uint64_t sz = depth + 1;
+                        if (sz < 64) atomicAdd(&lhist[sz], 1ull);
+                        else atomicAdd(&a.hist[sz < a.hist_len ? sz : a.hist_len - 1], 1ull);
+                    }
+                } else {
+                    // pivot over X0 candidates, then P0 candidates (position order)
+                    uint64_t best = 0;
+                    for (uint64_t c0 = 0; c0 < WX; c0 += kWave) {
+                        const uint64_t wx = (c0 + lane < WX) ? XX[c0 + lane] : 0;
+                        uint64_t nz = ballot(wx != 0);
+                        while (nz) {
+                            const uint32_t wi = __builtin_ctzll(nz);
+                            nz &= nz - 1;
+                            const uint64_t word = read_lane(wx, wi);
+                            if ((word >> lane) & 1) {
+                                const uint64_t k = (c0 + wi) * 64 + lane;
+                                const uint64_t* row = adjP + k * W;
+                                uint64_t sc = 0;
+                                for (uint64_t i = 0; i < W; i++) sc += __popcll(row[i] & read_lane(pw, (uint32_t)i));
+                                const uint64_t key = (sc << 32) | k;
+                                best = key > best ? key : best;
+                                if (ACCOUNT) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
+                            }
+                        }
+                    }
+                    {
+                        const uint64_t uw = pw | xpw;
+                        uint64_t nz = ballot(uw != 0);
+                        while (nz) {
+                            const uint32_t wi = __builtin_ctzll(nz);
+                            nz &= nz - 1;
+                            const uint64_t word = read_lane(uw, wi);
+                            if ((word >> lane) & 1) {
+                                const uint64_t k = s + (uint64_t)wi * 64 + lane;
+                                const uint64_t* row = adjP + k * W;
+                                uint64_t sc = 0;
+                                for (uint64_t i = 0; i < W; i++) sc += __popcll(row[i] & read_lane(pw, (uint32_t)i));
+                                const uint64_t key = (sc << 32) | k;
+                                best = key > best ? key : best;
+                                if (ACCOUNT) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
+                            }
+                        }
+                    }
+                    best = wave_max(best);
+                    const uint64_t pivot = best & 0xffffffffull;
+                    const uint64_t* prow = adjP + pivot * W;
+                    qw = lane < W ? (pw & ~prow[lane]) : 0;
+                    if (ACCOUNT) {
+                        if (lane == 0) my_bytes += cost_isect(np, degree_of(a.rp, C[pivot]));
+                        uint64_t nz = ballot(qw != 0);
+                        while (nz) {
+                            const uint32_t wi = __builtin_ctzll(nz);
+                            nz &= nz - 1;
+                            const uint64_t word = read_lane(qw, wi);
+                            if ((word >> lane) & 1)
+                                my_bytes += cost_isect(nPX, degree_of(a.rp, C[s + (uint64_t)wi * 64 + lane]));
+                        }
+                    }
+                }
+                fresh = false;
+            }
+            // next child: lowest q left in Q
+            const uint64_t nzq = ballot(qw != 0);
+            if (nzq != 0) {
+                const uint32_t wi = __builtin_ctzll(nzq);
+                const uint64_t word = read_lane(qw, wi);
+                const uint32_t b = __builtin_ctzll(word);
+                const uint64_t j = (uint64_t)wi * 64 + b;
+                if (lane == wi) {
+                    const uint64_t m = 1ull << b;
+                    qw &= ~m;
+                    pw &= ~m;
+                    xpw |= m;
+                }
+                if (lane < W) {
+                    F[lane] = pw;
+                    F[W + lane] = xpw;
+                    F[2 * W + lane] = qw;
+                }
+                const uint64_t* qrow = adjP + (s + j) * W;
+                if (lane < W) {
+                    const uint64_t r = qrow[lane];
+                    pw &= r;
+                    xpw &= r;
+                }
+                qw = 0;
+                const uint64_t* xrow = adjX + j * WX;
+                uint64_t* XXc = XX + FW;
+                for (uint64_t i = lane; i < WX; i += kWave) XXc[i] = XX[i] & xrow[i];
+                depth++;
+                fresh = true;
+                descend = true;
+            }
+            if (!descend) {
+                if (depth == 0) break;
+                depth--;
+                const uint64_t* Fp = frames + depth * FW;
+                if (lane < W) {
+                    pw = Fp[lane];
+                    xpw = Fp[W + lane];
+                    qw = Fp[2 * W + lane];
+                }
+            }
+        }
+        my_nodes += root_nodes;
+        if (a.per_root && lane == 0) a.per_root[si] = root_nodes;
+    }
+
+    // ---- flush per-wave and per-block tallies ----------------------------
+    if (ACCOUNT) my_bytes = wave_sum(my_bytes);
+    if (lane == 0) {
+        if (my_cliques) atomicAdd(&a.ctr[CTR_CLIQUES], (unsigned long long)my_cliques);
+        if (my_nodes) atomicAdd(&a.ctr[CTR_NODES], (unsigned long long)my_nodes);
+        if (ACCOUNT && my_bytes) atomicAdd(&a.ctr[CTR_BYTES], (unsigned long long)my_bytes);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x)
+        if (lhist[i]) atomicAdd(&a.hist[i < a.hist_len ? i : a.hist_len - 1], lhist[i]);
+}
+
+int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
+            uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
+    const bool account = flags & G2_RUN_ACCOUNT;
+    const uint32_t hl = hist ? hist_len : 2;
+    const uint32_t bw = c->tune.block_waves;
+    const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
+    const uint64_t nwaves = nblocks * bw;
+    const uint64_t sw = c->tune.wave_scratch_words;
+    int rc;
+    if ((rc = ensure(c, c->counters, CTR_N * 8, "counters"))) return rc;
+    if ((rc = ensure(c, c->hist, (uint64_t)hl * 8, "histogram"))) return rc;
+    if ((rc = ensure(c, c->list, std::max<uint64_t>(c->ns, 1) * 4, "deferred roots"))) return rc;
+    if ((rc = ensure(c, c->scratch, nwaves * sw * 8, "wave arenas"))) return rc;
+    if (per_root_nodes && (rc = ensure(c, c->per_root, std::max<uint64_t>(c->ns, 1) * 8, "per-root counts"))) return rc;
+    const uint32_t* order = nullptr;
+    if (sched && sched->order) {
+        if ((rc = ensure(c, c->order, std::max<uint64_t>(c->ns, 1) * 4, "schedule"))) return rc;
+        hipError_t e = hipMemcpyAsync(c->order.ptr, sched->order, c->ns * 4, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return hip_err(c, e, "H2D schedule");
+        order = (const uint32_t*)c->order.ptr;
+    }
+    hipError_t e;
+    auto* ctr = (unsigned long long*)c->counters.ptr;
+    if ((e = hipMemsetAsync(ctr, 0, CTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
+    if ((e = hipMemsetAsync(c->hist.ptr, 0, (uint64_t)hl * 8, c->stream))) return hip_err(c, e, "memset hist");
+    if (per_root_nodes && (e = hipMemsetAsync(c->per_root.ptr, 0, c->ns * 8, c->stream))) return hip_err(c, e, "memset per-root");
+
+    MceArgs a{};
+    a.rp = (const uint64_t*)c->rp.ptr;
+    a.cols = (const uint32_t*)c->cols.ptr;
+    a.src = (const uint32_t*)c->src.ptr;
+    a.order = order;
+    a.list = nullptr;
+    a.ntasks = c->ns;
+    a.scratch = (uint64_t*)c->scratch.ptr;
+    a.scratch_words = sw;
+    a.big_list = (uint32_t*)c->list.ptr;
+    a.ctr = ctr;
+    a.hist = (unsigned long long*)c->hist.ptr;
+    a.hist_len = hl;
+    a.per_root = per_root_nodes ? (unsigned long long*)c->per_root.ptr : nullptr;
+
+    auto launch = [&](uint64_t blocks) {
+        if (account) hipLaunchKernelGGL(mce_kernel<true>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);
+        else hipLaunchKernelGGL(mce_kernel<false>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);
+        return hipGetLastError();
+    };
+    float k1 = 0, k2 = 0, tot = 0;
+    uint32_t launches = 1;
+    if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
+    if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
+    if ((e = launch(nblocks))) return hip_err(c, e, "mce launch");
+    if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
+    unsigned long long h_ctr[CTR_N];
+    if ((e = hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
+    (void)hipEventElapsedTime(&k1, c->ev[1], c->ev[2]);
+    const uint64_t nbig = h_ctr[CTR_NBIG];
+    if (nbig) {
+        // large-root pass: arenas sized for the largest deferred root
+        const uint64_t need = h_ctr[CTR_MAXNEED];
+        uint64_t bwaves = std::min<uint64_t>(c->tune.big_waves, nbig);
+        while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
+        bwaves = (bwaves + bw - 1) / bw * bw;
+        if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
+        a.list = (const uint32_t*)c->list.ptr;
+        a.ntasks = nbig;
+        a.scratch = (uint64_t*)c->big_scratch.ptr;
+        a.scratch_words = need;
+        a.big_list = nullptr;
+        if ((e = hipMemsetAsync(ctr + CTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
+        if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
+        if ((e = launch(bwaves / bw))) return hip_err(c, e, "mce large-root launch");
+        if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
+        launches = 2;
+    }
+    if ((e = hipEventRecord(c->ev[3], c->stream))) return hip_err(c, e, "event");
+    if ((e = hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
+    if (hist && (e = hipMemcpyAsync(hist, c->hist.ptr, (uint64_t)hl * 8, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H hist");
+    if (per_root_nodes && c->ns &&
+        (e = hipMemcpyAsync(per_root_nodes, c->per_root.ptr, c->ns * 8, hipMemcpyDeviceToHost, c->stream)))
+        return hip_err(c, e, "D2H per-root");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
+    if (nbig) (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[3]);
+    if (h_ctr[CTR_ERR])
+        return set_err(c, G2_EBUDGET, "a root needs more than 4096 later neighbours or did not fit the large-root arena");
+    if (total) *total = h_ctr[CTR_CLIQUES];
+    if (st) {
+        st->kernel_ms = (double)k1 + (double)k2;
+        st->launches = launches;
+        st->total_ms = tot;
+        st->nodes = h_ctr[CTR_NODES];
+        st->alg_bytes = account ? h_ctr[CTR_BYTES] : 0;
+        st->big_roots = nbig;
+        st->max_scratch_words = h_ctr[CTR_MAXNEED];
+    }
+    return G2_OK;
+}
+
+}  // namespace g2

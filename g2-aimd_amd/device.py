@@ -1,5 +1,167 @@
-"""placeholder"""
-def hip_lib():
-    raise RuntimeError("libg2hip.so not built")
+"""Device side of the package: ctypes binding of libg2hip.so (include/g2hip.h).
+
+``Executor`` mirrors the reference's per-GPU PipelineExecutor
+(system/pipeline_executor.h:44-207): ``transfer`` uploads a view (CSR rows by
+global vertex id + source vertices, ``Transfer`` :121-149), ``run_mce`` is
+``Run`` with the BK app (app_BK/BK.h), ``set_plan`` + ``run_gm`` are ``Run``
+with the GMatch app (app_gmatch/gmatch.h).  Errors raise RuntimeError with the
+library's message (the reference prints and exits, device/cuda_context.h:18-25).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_hip = None
+
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+G2_RUN_ACCOUNT = 1
+
+# every symbol include/g2hip.h declares
+EXPORTS = ("g2_create", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
+           "g2_gm_set_plan", "g2_gm_run", "g2_device_info")
+
+
+class Tuning(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_uint32), ("block_waves", C.c_uint32),
+                ("wave_scratch_words", C.c_uint64), ("big_waves", C.c_uint32),
+                ("gm_chunk", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("launches", C.c_uint32), ("total_ms", C.c_double),
+                ("nodes", C.c_uint64), ("alg_bytes", C.c_uint64), ("big_roots", C.c_uint64),
+                ("max_scratch_words", C.c_uint64)]
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Sched(C.Structure):
+    _fields_ = [("order", C.c_void_p), ("n", C.c_uint64)]
+
+
+def hip_lib() -> C.CDLL:
+    """Load libg2hip.so (raises if it was not built: there is no fallback)."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    path = os.path.join(_HERE, "libg2hip.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    from . import G2Plan  # noqa: circular-safe (package already initialised)
+
+    vp = C.c_void_p
+    L.g2_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64, C.POINTER(Tuning)]
+    L.g2_destroy.argtypes = [vp]
+    L.g2_destroy.restype = None
+    L.g2_last_error.argtypes = [vp]
+    L.g2_last_error.restype = C.c_char_p
+    L.g2_upload_view.argtypes = [vp, u64p, C.c_uint64, u32p, C.c_uint64, u32p, C.c_uint64]
+    L.g2_mce_run.argtypes = [vp, C.POINTER(Sched), C.c_uint32, C.POINTER(C.c_uint64), vp,
+                             C.c_uint32, vp, C.POINTER(Stats)]
+    L.g2_gm_set_plan.argtypes = [vp, C.POINTER(G2Plan)]
+    L.g2_gm_run.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64), vp, C.POINTER(Stats)]
+    L.g2_device_info.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                 C.c_char_p, C.c_uint32]
+    _hip = L
+    return L
+
+
+def device_info(device: int = 0) -> dict:
+    cus, mem = C.c_uint32(), C.c_uint64()
+    name = C.create_string_buffer(64)
+    rc = hip_lib().g2_device_info(device, C.byref(cus), C.byref(mem), name, 64)
+    if rc != 0:
+        raise RuntimeError(f"g2_device_info failed ({rc})")
+    return {"num_cus": cus.value, "total_mem": mem.value, "arch": name.value.decode()}
+
+
 class Executor:
-    pass
+    """One device context (a consumer of the reference's worker pipeline)."""
+
+    def __init__(self, device: int = 0, mem_budget: int = 0, **tuning):
+        L = hip_lib()
+        t = Tuning(**tuning)
+        self._ctx = C.c_void_p()
+        rc = L.g2_create(C.byref(self._ctx), device, mem_budget, C.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"g2_create failed ({rc}): no usable GPU or bad arguments")
+        self.ns = 0
+
+    def close(self) -> None:
+        if self._ctx:
+            hip_lib().g2_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = hip_lib().g2_last_error(self._ctx).decode()
+            raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+    def transfer(self, g, sources=None) -> None:
+        """Upload a view (``Transfer``): CSR rows by global id + source ids."""
+        src = (np.arange(g.nv, dtype=np.uint32) if sources is None
+               else np.ascontiguousarray(sources, dtype=np.uint32))
+        cols = g.cols if g.ne else np.zeros(1, np.uint32)
+        s = src if len(src) else np.zeros(1, np.uint32)
+        self._check(hip_lib().g2_upload_view(self._ctx, g.row_ptrs, g.nv, cols, g.ne, s, len(src)),
+                    "g2_upload_view")
+        self.ns = len(src)
+
+    def run_mce(self, hist_len: int = 256, per_root: bool = False, order=None,
+                account: bool = False) -> dict:
+        """Maximal clique enumeration over the uploaded sources."""
+        total = C.c_uint64()
+        hist = np.zeros(hist_len, np.uint64)
+        prn = np.zeros(max(self.ns, 1), np.uint64) if per_root else None
+        st = Stats()
+        sched = None
+        if order is not None:
+            o = np.ascontiguousarray(order, dtype=np.uint32)
+            if len(o) != self.ns:
+                raise ValueError("order must be a permutation of the sources")
+            sched = Sched(o.ctypes.data, len(o))
+            self._order_keep = o
+        rc = hip_lib().g2_mce_run(self._ctx, C.byref(sched) if sched else None,
+                                  G2_RUN_ACCOUNT if account else 0, C.byref(total),
+                                  hist.ctypes.data, hist_len,
+                                  prn.ctypes.data if per_root else None, C.byref(st))
+        self._check(rc, "g2_mce_run")
+        out = {"total": total.value, "hist": hist, "stats": st.to_dict()}
+        if per_root:
+            out["per_root_nodes"] = prn[: self.ns]
+        return out
+
+    def set_plan(self, plan) -> None:
+        self._check(hip_lib().g2_gm_set_plan(self._ctx, C.byref(plan)), "g2_gm_set_plan")
+
+    def run_gm(self, per_root: bool = False, account: bool = False) -> dict:
+        cnt = C.c_uint64()
+        prc = np.zeros(max(self.ns, 1), np.uint64) if per_root else None
+        st = Stats()
+        rc = hip_lib().g2_gm_run(self._ctx, G2_RUN_ACCOUNT if account else 0, C.byref(cnt),
+                                 prc.ctypes.data if per_root else None, C.byref(st))
+        self._check(rc, "g2_gm_run")
+        out = {"count": cnt.value, "stats": st.to_dict()}
+        if per_root:
+            out["per_root_count"] = prc[: self.ns]
+        return out

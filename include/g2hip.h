@@ -1,0 +1,106 @@
+/* include/g2hip.h -- device C ABI of the MI355X subgraph-search library
+ * (libg2hip.so: gfx950 kernels + device runtime).
+ *
+ * This is the drop-in boundary that replaces the reference's per-GPU
+ * executor and its app hooks:
+ *   PipelineExecutor(dev, graph, max_sources, app, reserved, strategy)
+ *                                   system/pipeline_executor.h:44-99  -> g2_create
+ *   PipelineExecutor::Transfer(holder)  system/pipeline_executor.h:121-149 -> g2_upload_view
+ *   PipelineExecutor::Run() with BKExpandSequential
+ *                                   system/pipeline_executor.h:151-207,
+ *                                   app_BK/BK.h:323-479                 -> g2_mce_run
+ *   GMContext / allocateGMMemory    system/appbase.h:11-68,156-216      -> g2_gm_set_plan
+ *   PipelineExecutor::Run() with GMatchApp
+ *                                   app_gmatch/gmatch.h:67-674          -> g2_gm_run
+ *   total_counts_ reduction         system/worker.h:250-255,273-279     (counts are
+ *                                   reduced on the device and returned)
+ *
+ * Only plain integers and pointers cross it.  One context per consumer
+ * thread; a context is not thread-safe, several contexts may share a device
+ * (each has its own stream and memory budget).
+ *
+ * Return convention: 0 = OK, < 0 = error (G2_E*); g2_last_error() gives the
+ * message.  Host callers restore the reference's behaviour on error (print
+ * and exit(-1), device/cuda_context.h:18-25).
+ */
+#ifndef G2HIP_H
+#define G2HIP_H
+
+#include <stdint.h>
+#include "g2plan.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G2_OK          0
+#define G2_EINVAL     -1   /* bad argument */
+#define G2_EHIP       -2   /* HIP runtime error */
+#define G2_EBUDGET    -3   /* memory budget too small for the request */
+#define G2_ENOVIEW    -4   /* run before upload */
+#define G2_ENOPLAN    -5   /* matching run before a plan was set */
+
+typedef struct g2_ctx g2_ctx;
+
+/* launch and scratch parameters (0 = library default) */
+typedef struct g2_tuning {
+    uint32_t waves_per_cu;        /* resident search waves per CU of the persistent grid */
+    uint32_t block_waves;         /* waves per workgroup */
+    uint64_t wave_scratch_words;  /* per-wave u64 words for MCE local bitsets (small-root pass) */
+    uint32_t big_waves;           /* waves of the large-root pass */
+    uint32_t gm_chunk;            /* level-1 candidates per matching task (0 = whole root) */
+} g2_tuning;
+
+/* measurements of the last run */
+typedef struct g2_stats {
+    double   kernel_ms;           /* HIP-event time of the dominant search kernel, all launches */
+    uint32_t launches;            /* launches of the dominant search kernel */
+    double   total_ms;            /* HIP-event time of the whole run on the context's stream */
+    uint64_t nodes;               /* MCE search-tree nodes / GM partial-match extensions */
+    uint64_t alg_bytes;           /* modelled algorithmic bytes (only with G2_RUN_ACCOUNT) */
+    uint64_t big_roots;           /* roots routed to the large-root pass */
+    uint64_t max_scratch_words;   /* largest per-root scratch need seen */
+} g2_stats;
+
+/* root schedule for g2_mce_run (NULL = the given source order) */
+typedef struct g2_sched {
+    const uint32_t* order;        /* permutation of [0, ns): processing order of the sources */
+    uint64_t        n;            /* must equal the uploaded source count */
+} g2_sched;
+
+#define G2_RUN_ACCOUNT  1u        /* also compute the modelled algorithmic bytes */
+
+int  g2_create(g2_ctx** ctx, int device, uint64_t mem_budget_bytes, const g2_tuning* tuning);
+void g2_destroy(g2_ctx* ctx);
+const char* g2_last_error(const g2_ctx* ctx);
+
+/* Copy a view (CSR rows indexed by global vertex id, sorted) and its source
+ * vertices to the device.  The caller keeps ownership of the host arrays. */
+int  g2_upload_view(g2_ctx* ctx, const uint64_t* row_ptrs, uint64_t nv,
+                    const uint32_t* cols, uint64_t ne,
+                    const uint32_t* sources, uint64_t ns);
+
+/* Maximal clique enumeration rooted at the uploaded sources (pivoted
+ * Bron-Kerbosch, app_BK/BK.h).  total = number of maximal cliques;
+ * hist[k] = cliques with k vertices (k >= hist_len-1 land in the last bin);
+ * per_root_nodes (nullable, ns entries, source order) = search-tree nodes of
+ * each root's subtree. */
+int  g2_mce_run(g2_ctx* ctx, const g2_sched* sched, uint32_t flags,
+                uint64_t* total, uint64_t* hist, uint32_t hist_len,
+                uint64_t* per_root_nodes, g2_stats* stats);
+
+/* Set the query plan for g2_gm_run (tables of system/plan.h). */
+int  g2_gm_set_plan(g2_ctx* ctx, const g2_plan* plan);
+
+/* Match count of the plan rooted at the uploaded sources (u64, the
+ * overflow-free count of app_gmatch/gmatch.h).  per_root_count nullable. */
+int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_count,
+               g2_stats* stats);
+
+/* Device properties used for launch sizing */
+int  g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* name, uint32_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+"""GPU subgraph matching (libg2hip.so g2_gm_run) against the oracle: u64
+counts and per-root counts, bit-exact."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+PRESETS = list(range(0, 19)) + [23, 24, 25, 26, 27]
+
+
+def both(g2, orc, ex, g, qadj, sources=None, account=False):
+    p = g2.plan_build(qadj)
+    ex.transfer(g, sources)
+    ex.set_plan(p)
+    got = ex.run_gm(per_root=True, account=account)
+    ref = orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)), sources=sources, per_root=True)
+    assert got["count"] == ref["count"]
+    assert np.array_equal(got["per_root_count"], ref["per_root_count"])
+    if account:
+        assert got["stats"]["alg_bytes"] == ref["bytes"]
+    return got
+
+
+@pytest.fixture(scope="module")
+def rmat12(g2):
+    return g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
+
+
+@pytest.fixture(scope="module")
+def rmat9(g2):
+    return g2.gen_rmat(9, 8, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
+
+
+@pytest.mark.parametrize("q", PRESETS)
+def test_presets_rmat(g2, orc, gpu_exec, rmat9, q):
+    print("case: preset", q, flush=True)
+    _, qadj = g2.query_preset(q)
+    both(g2, orc, gpu_exec, rmat9, qadj, account=True)
+
+
+def test_random_patterns(g2, orc, gpu_exec):
+    print("case: random patterns", flush=True)
+    with open(os.path.join(HERE, "golden", "plans.json")) as f:
+        fixtures = json.load(f)["random"]
+    g = G.gnp(g2, 300, 0.05, 8)
+    for item in fixtures:
+        both(g2, orc, gpu_exec, g, g2.query_remap(item["qadj"]))
+
+
+def test_clique_closed_forms(g2, orc, gpu_exec):
+    print("case: clique closed forms", flush=True)
+    g = G.complete(g2, 30)
+    for q, k in ((0, 3), (24, 4), (25, 5), (26, 6)):
+        _, qadj = g2.query_preset(q)
+        assert both(g2, orc, gpu_exec, g, qadj)["count"] == math.comb(30, k)
+
+
+def test_triangles_4cliques_rmat15(g2, orc, gpu_exec):
+    print("case: rmat-15 q0 q24", flush=True)
+    g = g2.gen_rmat(15, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
+    for q in (0, 24):
+        _, qadj = g2.query_preset(q)
+        both(g2, orc, gpu_exec, g, qadj, account=True)
+
+
+def test_windows_and_views(g2, orc, rmat12):
+    """Level-1 windows split hub roots across waves; hop-2 views of 4 bins add
+    up to the whole graph."""
+    print("case: windows and views", flush=True)
+    _, qadj = g2.query_preset(24)
+    whole = orc.gm(rmat12, orc.OrcPlan.from_buffer_copy(bytes(g2.plan_build(qadj))))["count"]
+    with g2.Executor(device=0, gm_chunk=16) as ex:
+        both(g2, orc, ex, rmat12, qadj)
+        tot = 0
+        bins = np.arange(rmat12.nv) % 4
+        for b in range(4):
+            src = np.nonzero(bins == b)[0].astype(np.uint32)
+            view = g2.view_materialize(rmat12, src, 2, 3)
+            tot += both(g2, orc, ex, view, qadj, sources=src)["count"]
+        assert tot == whole

@@ -1,0 +1,101 @@
+"""GPU maximal clique enumeration (libg2hip.so g2_mce_run) against the oracle:
+totals, size histograms and per-root search-tree node counts, bit-exact."""
+import numpy as np
+import pytest
+
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+CFG1_SEED = 0x6732A1D0C0000001
+
+
+def check(ex, orc, g, sources=None, **kw):
+    ex.transfer(g, sources)
+    got = ex.run_mce(per_root=True, **kw)
+    ref = orc.mce(g, sources=sources, per_root=True)
+    assert got["total"] == ref["total"]
+    assert np.array_equal(got["hist"], ref["hist"])
+    assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+    assert got["stats"]["nodes"] == ref["nodes"]
+    return got, ref
+
+
+def test_small_graphs(g2, orc, gpu_exec):
+    print("case: small graphs", flush=True)
+    for seed, (n, p) in enumerate([(8, 0.5), (30, 0.3), (60, 0.5), (150, 0.1), (100, 0.9)]):
+        check(gpu_exec, orc, G.gnp(g2, n, p, seed))
+    for g in (G.complete(g2, 20), G.complete_bipartite(g2, 9, 13), G.cycle(g2, 40),
+              G.moon_moser(g2, 7), G.turan(g2, [5, 1, 7, 3])):
+        check(gpu_exec, orc, g)
+
+
+def test_wide_neighbourhoods(g2, orc, gpu_exec):
+    # p0 > 64 exercises multi-word P sets; dense graphs give deep trees
+    print("case: wide neighbourhoods", flush=True)
+    check(gpu_exec, orc, G.gnp(g2, 300, 0.5, 11))
+    check(gpu_exec, orc, G.moon_moser(g2, 9))
+
+
+def test_empty_and_isolated(g2, orc, gpu_exec):
+    print("case: empty and isolated", flush=True)
+    g = g2.Graph(np.zeros(6, np.uint64), np.zeros(0, np.uint32))
+    got, _ = check(gpu_exec, orc, g)
+    assert got["total"] == 0
+    check(gpu_exec, orc, G.gnp(g2, 40, 0.1, 3), sources=np.array([], np.uint32))
+
+
+@pytest.fixture(scope="module")
+def cfg1(g2):
+    g = g2.gen_chung_lu(100000, 2.3, 2.0, 2000.0, 1000000, CFG1_SEED).preprocess()
+    g, order, shells, dg = g.kcore_recode()
+    return g, shells
+
+
+def test_chung_lu_100k(g2, orc, gpu_exec, cfg1):
+    print("case: chung-lu 100k", flush=True)
+    g, _ = cfg1
+    got, ref = check(gpu_exec, orc, g, account=True)
+    assert got["stats"]["alg_bytes"] == ref["bytes"]
+
+
+def test_chung_lu_views(g2, orc, gpu_exec, cfg1):
+    """4 kcore round-robin bins as hop-2 views: per-view results equal the
+    oracle on the same view and add up to the whole graph's."""
+    print("case: chung-lu views", flush=True)
+    g, shells = cfg1
+    bins = g2.kcore_bins(shells, 4, g.nv)
+    total = 0
+    hist = np.zeros(256, np.uint64)
+    for b in range(4):
+        src = np.nonzero(bins == b)[0].astype(np.uint32)
+        view = g2.view_materialize(g, src, 2)
+        got, _ = check(gpu_exec, orc, view, sources=src)
+        total += got["total"]
+        hist += got["hist"]
+    whole = orc.mce(g)
+    assert total == whole["total"] and np.array_equal(hist, whole["hist"])
+
+
+def test_large_root_pass_and_schedule(g2, orc):
+    """Tiny per-wave arenas force every root through the large-root pass; a
+    reversed schedule changes the order, never the results."""
+    print("case: large-root pass", flush=True)
+    g = G.gnp(g2, 400, 0.08, 21)
+    ref = orc.mce(g, per_root=True)
+    with g2.Executor(device=0, wave_scratch_words=64, big_waves=64) as ex:
+        ex.transfer(g)
+        got = ex.run_mce(per_root=True)
+        assert got["stats"]["big_roots"] > 0
+        assert got["total"] == ref["total"] and np.array_equal(got["hist"], ref["hist"])
+        assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+        got = ex.run_mce(per_root=True, order=np.arange(g.nv, dtype=np.uint32)[::-1].copy())
+        assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+
+
+def test_rmat_mce(g2, orc, gpu_exec):
+    """R-MAT hubs: neighbourhoods of thousands of vertices (large-root pass)."""
+    print("case: rmat-12 mce", flush=True)
+    g = g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000005).preprocess()
+    g, _, _, _ = g.kcore_recode()
+    check(gpu_exec, orc, g, account=True)

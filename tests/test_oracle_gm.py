@@ -1,0 +1,79 @@
+"""Pins the matching oracle (orc_gm, a restatement of app_gmatch/gmatch.h)
+against brute-force injective homomorphisms / |Aut|, closed forms and an
+independent k-clique counter."""
+import json
+import math
+import os
+
+import pytest
+
+import graphs as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PRESETS = list(range(0, 19)) + [23, 24, 25, 26, 27]
+
+
+def orc_plan(g2, orc, qadj):
+    p = g2.plan_build(qadj)
+    return orc.OrcPlan.from_buffer_copy(bytes(p))
+
+
+def bfs_order(qadj):
+    n = len(qadj)
+    order, seen = [0], {0}
+    while len(order) < n:
+        for u in list(order):
+            for v in range(n):
+                if qadj[u] >> v & 1 and v not in seen:
+                    seen.add(v)
+                    order.append(v)
+    pos = {v: i for i, v in enumerate(order)}
+    out = [0] * n
+    for v in range(n):
+        for w in range(n):
+            if qadj[v] >> w & 1:
+                out[pos[v]] |= 1 << pos[w]
+    return out
+
+
+@pytest.mark.parametrize("q", PRESETS)
+def test_presets_vs_bruteforce(g2, orc, q):
+    g = G.gnp(g2, 26 if q < 19 else 30, 0.35, 100 + q)
+    n, qadj = g2.query_preset(q)
+    homs = orc.bruteforce_pattern(g, bfs_order(qadj))
+    aut = orc.automorphism_count(qadj)
+    assert homs % aut == 0
+    assert orc.gm(g, orc_plan(g2, orc, qadj))["count"] == homs // aut
+
+
+def test_random_patterns_vs_bruteforce(g2, orc):
+    with open(os.path.join(HERE, "golden", "plans.json")) as f:
+        fixtures = json.load(f)["random"]
+    g = G.gnp(g2, 24, 0.4, 7)
+    for item in fixtures[:24]:
+        qadj = g2.query_remap(item["qadj"])
+        homs = orc.bruteforce_pattern(g, bfs_order(qadj))
+        aut = orc.automorphism_count(qadj)
+        assert orc.gm(g, orc_plan(g2, orc, qadj))["count"] == homs // aut, item["seed"]
+
+
+@pytest.mark.parametrize("n", [3, 6, 11])
+def test_cliques_closed_form(g2, orc, n):
+    g = G.complete(g2, n)
+    for q, k in ((0, 3), (24, 4), (25, 5)):
+        _, qadj = g2.query_preset(q)
+        assert orc.gm(g, orc_plan(g2, orc, qadj))["count"] == math.comb(n, k)
+
+
+def test_kclique_independent_method(g2, orc):
+    g = g2.gen_rmat(11, 16, 0.57, 0.19, 0.19, 77).preprocess()
+    for q, k in ((0, 3), (24, 4), (25, 5)):
+        _, qadj = g2.query_preset(q)
+        assert orc.gm(g, orc_plan(g2, orc, qadj))["count"] == orc.kclique_count(g, k)
+
+
+def test_per_root_counts_sum(g2, orc):
+    g = G.gnp(g2, 60, 0.2, 5)
+    _, qadj = g2.query_preset(24)
+    r = orc.gm(g, orc_plan(g2, orc, qadj), per_root=True)
+    assert int(r["per_root_count"].sum()) == r["count"]
