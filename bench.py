@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""bench.py -- headline benchmark of the MI355X subgraph-search hot path.
+
+Workload (N=1): 4-clique matching (preset q=24, the reference's `run -q 24`)
+over every root of a seeded R-MAT graph, scale 22, edge factor 16,
+(a,b,c,d) = (.57,.19,.19,.05), symmetrised, deduplicated, self loops and
+degree-0 vertices removed -- the graph of BASELINE config "R-MAT scale 22"
+resident in HBM as one view.  One step = one full g2_gm_run over all roots.
+Also reported: triangles (q=0) on the same graph and maximal cliques on the
+Chung-Lu n=100k graph (secondary lines inside the JSON object).
+
+value      = 4-clique matches / second of the timed steps (inputs resident in HBM)
+roofline   = modelled algorithmic bytes of one launch / its HIP-event duration
+             (bytes from the kernel's own accounting mode, computed once,
+             untimed; the byte model is the one in oracle/oracle.cpp)
+cpu_baseline = the oracle (C++/OpenMP restatement of app_gmatch/gmatch.h) on a
+             seeded random sample of roots, timed on this host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "max cliques/s + 4-clique matches/s, 1 MI355X; learned-vs-plain sched speedup"
+HBM_PEAK_GBS = 8000.0
+RMAT_SEED = 0x6732A1D0C0000002
+CL_SEED = 0x6732A1D0C0000001
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_rmat(g2, scale):
+    t = time.time()
+    g = g2.gen_rmat(scale, 16, 0.57, 0.19, 0.19, RMAT_SEED).preprocess()
+    log(f"rmat-{scale}: V={g.nv} E={g.ne // 2} undirected, max degree {int(g.degrees().max())}, "
+        f"{time.time() - t:.1f}s to generate")
+    return g
+
+
+def time_gm(ex, steps, warmup):
+    for _ in range(warmup):
+        ex.run_gm()
+    ms, kms, cnt = [], [], None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        r = ex.run_gm()
+        ms.append((time.perf_counter() - t0) * 1e3)
+        kms.append(r["stats"]["kernel_ms"])
+        if cnt is not None and r["count"] != cnt:
+            raise RuntimeError("count changed between steps")
+        cnt = r["count"]
+        log(f"step {i}: {r['count']} matches, kernel {kms[-1]:.2f} ms, wall {ms[-1]:.2f} ms")
+    return cnt, float(np.mean(ms)), float(np.mean(kms))
+
+
+def cpu_baseline(orc, g, plan, target_s, seed=7):
+    """oracle on a seeded random root sample sized to ~target_s seconds"""
+    threads = os.cpu_count() or 1
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(g.nv).astype(np.uint32)
+    frac = 0.002
+    while True:
+        n = max(1, int(frac * g.nv))
+        t0 = time.perf_counter()
+        r = orc.gm(g, plan, sources=np.sort(perm[:n]), threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= target_s * 0.5 or n == g.nv:
+            break
+        frac = min(1.0, frac * max(2.0, target_s / max(dt, 1e-3)))
+    return {"value": r["count"] / dt, "unit": "4-clique matches/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} of {g.nv} roots (seeded uniform sample), {r['count']} matches in {dt:.2f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22, help="R-MAT scale of the workload")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--gm-chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "the hot path shards by root ranges with one host-summed u64 per GPU, "
+              "see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    g2 = importlib.import_module("g2-aimd_amd")
+    from importlib import import_module
+    dev = import_module("g2-aimd_amd.device")
+    info = dev.device_info(0)
+    log(f"device: {info}")
+
+    g = build_rmat(g2, args.scale)
+    ex = g2.Executor(device=0, gm_chunk=args.gm_chunk)
+    t0 = time.perf_counter()
+    ex.transfer(g)
+    upload_ms = (time.perf_counter() - t0) * 1e3
+    plan24 = g2.plan_for_preset(24)
+    ex.set_plan(plan24)
+
+    # modelled bytes of one launch (accounting variant, untimed)
+    acc = ex.run_gm(account=True)
+    alg_bytes = acc["stats"]["alg_bytes"]
+    log(f"4-clique count {acc['count']}, modelled bytes/launch {alg_bytes}")
+
+    cnt, ms_step, kernel_ms = time_gm(ex, args.steps, args.warmup)
+    assert cnt == acc["count"]
+    value = cnt / (ms_step / 1e3)
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "4-clique matches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 vertex ids, u64 counts (integer set intersection)",
+        "data": "synthetic (seeded R-MAT, generated on the host)",
+        "config": {
+            "workload": f"4-clique matching (q=24) over all roots of R-MAT scale {args.scale}, "
+                        f"edge factor 16, (.57,.19,.19,.05), one resident view",
+            "vertices": g.nv, "undirected_edges": g.ne // 2, "matches": cnt,
+            "gm_chunk": args.gm_chunk,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "gm_kernel<false>", "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes,
+        },
+        "upload_ms": upload_ms,
+    }
+
+    if not args.no_extra:
+        extra = {}
+        ex.set_plan(g2.plan_for_preset(0))
+        tri, tri_ms, tri_kms = time_gm(ex, 1, 1)
+        extra["triangles"] = {"count": tri, "ms": tri_ms, "per_s": tri / (tri_ms / 1e3)}
+        ex.close()
+        cl = g2.gen_chung_lu(100000, 2.3, 2.0, 2000.0, 1000000, CL_SEED).preprocess()
+        cl, _, _, _ = cl.kcore_recode()
+        with g2.Executor(device=0) as mx:
+            mx.transfer(cl)
+            r = mx.run_mce()
+            acc_m = mx.run_mce(account=True)
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                r = mx.run_mce()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            kms = r["stats"]["kernel_ms"]
+            extra["mce_chung_lu_100k"] = {
+                "max_cliques": r["total"], "ms": float(np.mean(ts)), "kernel_ms": kms,
+                "per_s": r["total"] / (np.mean(ts) / 1e3),
+                "alg_bytes": acc_m["stats"]["alg_bytes"],
+                "achieved_GBs": acc_m["stats"]["alg_bytes"] / (kms / 1e3) / 1e9,
+            }
+        out["extra"] = extra
+    else:
+        ex.close()
+
+    if not args.no_cpu:
+        import oracle
+        po = oracle.OrcPlan.from_buffer_copy(bytes(plan24))
+        out["cpu_baseline"] = cpu_baseline(oracle, g, po, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,22 @@ int g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* nam
     return G2_OK;
 }
 
+int g2_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void* g2_pinned_alloc(uint64_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+void g2_pinned_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
 int g2_create(g2_ctx** out, int device, uint64_t budget, const g2_tuning* t) {
     if (!out) return G2_EINVAL;
     *out = nullptr;
@@ -151,6 +167,8 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
     if (!c) return G2_EINVAL;
     if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
     if (hist && hist_len < 2) return set_err(c, G2_EINVAL, "hist_len must be >= 2");
+    if (sched && (sched->begin > sched->end || sched->end > c->ns))
+        return set_err(c, G2_EINVAL, "schedule range outside the sources");
     if (sched && sched->order) {
         if (sched->n != c->ns) return set_err(c, G2_EINVAL, "schedule length != source count");
         std::vector<uint8_t> seen(c->ns, 0);

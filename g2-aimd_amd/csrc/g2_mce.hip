@@ -40,7 +40,8 @@ struct MceArgs {
     const uint32_t* __restrict__ src;
     const uint32_t* __restrict__ order;  // task -> source index (nullable)
     const uint32_t* __restrict__ list;   // task -> source index of the large-root pass (nullable)
-    uint64_t ntasks;
+    uint64_t first;                      // first task (range runs)
+    uint64_t ntasks;                     // one past the last task
     uint64_t* scratch;
     uint64_t scratch_words;              // per wave
     uint32_t* big_list;                  // deferred source indices (null in the large-root pass)
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
     while (true) {
         unsigned long long t = 0;
         if (lane == 0) t = atomicAdd(&a.ctr[CTR_NEXT], 1ull);
-        t = uniform((uint64_t)t);
+        t = uniform((uint64_t)t) + a.first;
         if (t >= a.ntasks) break;
         const uint32_t si = uniform(a.list ? a.list[t] : (a.order ? a.order[t] : (uint32_t)t));
         const uint32_t v = uniform(a.src[si]);
@@ -351,7 +352,12 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     a.src = (const uint32_t*)c->src.ptr;
     a.order = order;
     a.list = nullptr;
+    a.first = 0;
     a.ntasks = c->ns;
+    if (sched && sched->end > sched->begin) {
+        a.first = sched->begin;
+        a.ntasks = sched->end;
+    }
     a.scratch = (uint64_t*)c->scratch.ptr;
     a.scratch_words = sw;
     a.big_list = (uint32_t*)c->list.ptr;
@@ -384,6 +390,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         bwaves = (bwaves + bw - 1) / bw * bw;
         if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
         a.list = (const uint32_t*)c->list.ptr;
+        a.first = 0;
         a.ntasks = nbig;
         a.scratch = (uint64_t*)c->big_scratch.ptr;
         a.scratch_words = need;

@@ -24,7 +24,8 @@ G2_RUN_ACCOUNT = 1
 
 # every symbol include/g2hip.h declares
 EXPORTS = ("g2_create", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
-           "g2_gm_set_plan", "g2_gm_run", "g2_device_info")
+           "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
+           "g2_pinned_free")
 
 
 class Tuning(C.Structure):
@@ -43,7 +44,8 @@ class Stats(C.Structure):
 
 
 class Sched(C.Structure):
-    _fields_ = [("order", C.c_void_p), ("n", C.c_uint64)]
+    _fields_ = [("order", C.c_void_p), ("n", C.c_uint64), ("begin", C.c_uint64),
+                ("end", C.c_uint64)]
 
 
 def hip_lib() -> C.CDLL:
@@ -139,7 +141,7 @@ class Executor:
             o = np.ascontiguousarray(order, dtype=np.uint32)
             if len(o) != self.ns:
                 raise ValueError("order must be a permutation of the sources")
-            sched = Sched(o.ctypes.data, len(o))
+            sched = Sched(o.ctypes.data, len(o), 0, 0)
             self._order_keep = o
         rc = hip_lib().g2_mce_run(self._ctx, C.byref(sched) if sched else None,
                                   G2_RUN_ACCOUNT if account else 0, C.byref(total),

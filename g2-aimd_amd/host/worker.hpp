@@ -1,0 +1,183 @@
+// host/worker.hpp -- the reference's worker pipeline over the two C ABIs.
+//
+// Same command line, same producer/consumer structure and the same output
+// lines as Worker<App>::run (system/worker.h:28-280):
+//   * CommandLine flags -dg -q -m -t -qs -h -pn -cn -dr -ds -dst -ss
+//     (system/worker.h:31-43, common/command_line.h);
+//   * the .bin graph, the <graph>.<hop>.hop.vbmap bin map, sort / reorder /
+//     split of the bins (view/view_bin_manager.h:34-189);
+//   * pn producer threads materialise bins into qs pinned holders
+//     (view/view_bin.h:61-159), cn consumer threads pull holders over two
+//     bounded queues (released / assigned) and run the app on the device.
+// Differences by design:
+//   * consumers are not limited to one per GPU: consumer c uses device
+//     c % #GPUs with an explicit memory budget (the reference grabs all free
+//     memory per executor, system/appbase.h:93-114);
+//   * `bk` prints the summed maximal-clique count as "Total counts" (the
+//     reference's device code never writes the counters that line sums, so it
+//     prints 0 there) and the clique-size histogram.
+#pragma once
+#include <atomic>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <iostream>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <sys/time.h>
+#include <thread>
+#include <vector>
+
+#include "g2hip.h"
+#include "g2host.h"
+
+namespace g2w {
+
+// ---- command line (common/command_line.h semantics) ----------------------
+struct CommandLine {
+    int argc;
+    char** argv;
+    std::string get(const std::string& opt, const std::string& def) const {
+        for (int i = 1; i + 1 < argc; i++)
+            if (opt == argv[i]) return argv[i + 1];
+        return def;
+    }
+    int get_int(const std::string& opt, int def) const {
+        for (int i = 1; i + 1 < argc; i++)
+            if (opt == argv[i]) return std::atoi(argv[i + 1]);
+        return def;
+    }
+    double get_double(const std::string& opt, double def) const {
+        for (int i = 1; i + 1 < argc; i++)
+            if (opt == argv[i]) return std::atof(argv[i + 1]);
+        return def;
+    }
+};
+
+// ---- "finish <tag>, elapsed_time=<ms>ms" (common/timer.h) -------------------
+struct Timer {
+    double t0 = 0, t1 = 0;
+    static double now_us() {
+        timeval tv;
+        gettimeofday(&tv, nullptr);
+        return tv.tv_sec * 1e6 + tv.tv_usec;
+    }
+    void start() { t0 = now_us(); }
+    void stop() { t1 = now_us(); }
+    double ms() const { return (t1 - t0) / 1000.0; }
+    std::string line(const std::string& tag) const {
+        std::ostringstream o;
+        o << std::fixed << "finish " << tag << ", elapsed_time=" << ms() << "ms";
+        return o.str();
+    }
+};
+
+// whole lines go to stdout atomically across threads
+inline void say(const std::string& s) {
+    static std::mutex m;
+    std::lock_guard<std::mutex> g(m);
+    std::cout << s << std::endl;
+}
+
+[[noreturn]] inline void die(const std::string& s) {
+    say(s);
+    std::exit(-1);
+}
+
+// ---- bounded blocking queue (the MPMC hand-off of holder ids) --------------
+class BoundedQueue {
+   public:
+    explicit BoundedQueue(size_t cap) : cap_(cap) {}
+    void push(int v) {
+        std::unique_lock<std::mutex> l(m_);
+        not_full_.wait(l, [&] { return q_.size() < cap_; });
+        q_.push_back(v);
+        not_empty_.notify_one();
+    }
+    void pop(int& v) {
+        std::unique_lock<std::mutex> l(m_);
+        not_empty_.wait(l, [&] { return !q_.empty(); });
+        v = q_.front();
+        q_.pop_front();
+        not_full_.notify_one();
+    }
+    bool try_pop(int& v) {
+        std::lock_guard<std::mutex> l(m_);
+        if (q_.empty()) return false;
+        v = q_.front();
+        q_.pop_front();
+        not_full_.notify_one();
+        return true;
+    }
+
+   private:
+    size_t cap_;
+    std::deque<int> q_;
+    std::mutex m_;
+    std::condition_variable not_full_, not_empty_;
+};
+
+// ---- view bins -------------------------------------------------------------
+struct ViewBin {
+    int id;
+    std::vector<uint32_t> sources;
+    uint64_t factor(const uint64_t* rp) const {  // sum of source degrees
+        uint64_t s = 0;
+        for (uint32_t v : sources) s += rp[v + 1] - rp[v];
+        return s;
+    }
+};
+
+struct Holder {  // pinned materialised view (view/view_bin_holder.h)
+    uint64_t nv = 0, cap = 0, ne = 0;
+    uint64_t* rp = nullptr;
+    uint32_t* cols = nullptr;
+    int view_id = -1;
+    std::vector<uint32_t> sources;
+    Holder(uint64_t nv_, uint64_t cap_) : nv(nv_), cap(cap_) {
+        rp = (uint64_t*)g2_pinned_alloc((nv + 1) * 8);
+        cols = (uint32_t*)g2_pinned_alloc(std::max<uint64_t>(cap, 1) * 4);
+        if (!rp || !cols) die("cudaMallocHost failed: pinned view holder");
+    }
+    ~Holder() {
+        g2_pinned_free(rp);
+        g2_pinned_free(cols);
+    }
+};
+
+std::vector<ViewBin> load_bins(const std::string& file, uint64_t nv, size_t* max_sources);
+void split_bins(std::vector<ViewBin>& pool, const uint64_t* rp, int device_num, int times);
+void reorder_bins(std::vector<ViewBin>& pool, const uint64_t* rp);
+
+// ---- per-consumer device executor (system/pipeline_executor.h) -------------
+struct RunResult {
+    uint64_t count = 0;                 // GM matches / BK maximal cliques of this view
+    std::vector<uint64_t> hist;         // BK clique sizes
+};
+
+class Executor {
+   public:
+    Executor(int consumer, int device, uint64_t budget);
+    ~Executor();
+    void transfer(const Holder& h);
+    RunResult run_bk(uint32_t hist_len);
+    RunResult run_gm();
+    void set_plan(const g2_plan& p);
+
+   private:
+    void check(int rc, const char* what);
+    int consumer_;
+    std::string tag_;
+    g2_ctx* ctx_ = nullptr;
+    uint64_t ns_ = 0;
+};
+
+// ---- the worker ------------------------------------------------------------
+enum class App { BK, GM };
+int run_worker(int argc, char** argv, App app);
+
+}  // namespace g2w

@@ -66,6 +66,7 @@ typedef struct g2_stats {
 typedef struct g2_sched {
     const uint32_t* order;        /* permutation of [0, ns): processing order of the sources */
     uint64_t        n;            /* must equal the uploaded source count */
+    uint64_t        begin, end;   /* process tasks [begin, end) of that order only (0,0 = all) */
 } g2_sched;
 
 #define G2_RUN_ACCOUNT  1u        /* also compute the modelled algorithmic bytes */
@@ -99,6 +100,13 @@ int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_
 
 /* Device properties used for launch sizing */
 int  g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* name, uint32_t name_len);
+/* Number of visible GPUs (cudaGetDeviceCount in system/worker.h:45-48). */
+int  g2_device_count(void);
+
+/* Page-locked host memory for view holders (cudaMallocHost in
+ * view/view_bin_holder.h:28-29); NULL on failure. */
+void* g2_pinned_alloc(uint64_t bytes);
+void  g2_pinned_free(void* p);
 
 #ifdef __cplusplus
 }

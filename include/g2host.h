@@ -108,6 +108,19 @@ int  g2h_view_materialize(const g2h_graph* g, const uint32_t* sources, uint64_t 
                           uint64_t root_degree, uint64_t* row_ptrs_out, uint32_t* cols_out,
                           uint64_t cols_cap, uint64_t* ne_out);
 
+/* ---- AIMD root-chunk controller (system/subgraph_container.h:178-211) ----
+ * chunk starts at 1e6 roots; an overflowing iteration divides it by 8 (not
+ * below 1) and sets the additive step to max(5% of the new chunk, 1) and the
+ * threshold to max(2000, old chunk / 2); a clean iteration below the
+ * threshold adds `step` and doubles it, at or above it adds the fixed step. */
+typedef struct g2h_aimd {
+    uint32_t chunk, step, add_step, threshold;
+    int32_t  recover;
+} g2h_aimd;
+void g2h_aimd_init(g2h_aimd* a);
+/* returns the reference's trace prefix: "--- chunk: ", "+++ chunk: " or "... chunk: " */
+const char* g2h_aimd_adjust(g2h_aimd* a, int overflow);
+
 const char* g2h_last_error(void);
 
 #ifdef __cplusplus
