@@ -254,6 +254,10 @@ __global__ __launch_bounds__(256) void gm_kernel(GmArgs a) {
             auto build_pre = [&](uint32_t id, bool charge) {
                 // prefix = positions < id-1 are matched
                 const GmStep& st = a.prog.step[id];
+                if (!ACCOUNT && st.pre_is_prev) {  // PRE(id) is L(id-1), already built
+                    wave_sync();
+                    return;
+                }
                 const uint32_t parent = min_row(a, S, st.npre, st.pre);
                 const uint32_t pm = S.M[parent];
                 const Range rg = cond_range(S, st.npcond, st.pcop, st.pcpos);
@@ -283,10 +287,13 @@ __global__ __launch_bounds__(256) void gm_kernel(GmArgs a) {
                 nodes++;
                 uint64_t bsum = 0;
                 if (st.shared) {
+                    const bool alias = !ACCOUNT && st.pre_is_prev;
+                    const uint32_t* pl = alias ? L(id) : PRE(nx);
+                    const uint64_t plen = alias ? S.len[id] : S.prelen[nx];
                     if (nx + 1 == n) {
-                        filter_pre<ACCOUNT>(a, S, st, PRE(nx), S.prelen[nx], nullptr, &lane_total, &bsum);
+                        filter_pre<ACCOUNT>(a, S, st, pl, plen, nullptr, &lane_total, &bsum);
                     } else {
-                        const uint64_t nout = filter_pre<ACCOUNT>(a, S, st, PRE(nx), S.prelen[nx], L(nx), nullptr, &bsum);
+                        const uint64_t nout = filter_pre<ACCOUNT>(a, S, st, pl, plen, L(nx), nullptr, &bsum);
                         if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
                         if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
                     }
@@ -378,6 +385,26 @@ int gm_compile(g2_ctx* c, const g2_plan* p) {
         s.store_prefix = (id + 1 < n && p->strategy[id + 1] == 1) ? 1 : 0;
     }
     if (g.step[1].shared) return set_err(c, G2_EINVAL, "position 1 cannot be shared");
+    // A shared position whose pre back neighbours and pre conditions are, as
+    // sets, the back neighbours and conditions of the previous (non-shared)
+    // position filters exactly the same candidates: reuse that list.
+    for (uint32_t id = 2; id < n; id++) {
+        GmStep& s = g.step[id];
+        const GmStep& prev = g.step[id - 1];
+        if (!s.shared || prev.shared || s.npre != prev.nbn || s.npcond != prev.ncond) continue;
+        uint32_t mb = 0, mp = 0;
+        for (uint32_t k = 0; k < s.npre; k++) mb |= 1u << s.pre[k];
+        for (uint32_t k = 0; k < prev.nbn; k++) mp |= 1u << prev.bn[k];
+        if (mb != mp) continue;
+        bool same = true;
+        for (uint32_t k = 0; k < s.npcond && same; k++) {
+            bool found = false;
+            for (uint32_t l = 0; l < prev.ncond; l++)
+                found |= prev.cop[l] == s.pcop[k] && prev.cpos[l] == s.pcpos[k];
+            same = found;
+        }
+        s.pre_is_prev = same ? 1 : 0;
+    }
     c->prog = g;
     c->plan = *p;
     c->has_plan = true;

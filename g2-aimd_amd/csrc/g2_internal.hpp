@@ -30,6 +30,7 @@ struct GmStep {
     uint32_t nafter, after[G2_MAXQ];
     uint32_t nacond, acop[G2_MAXQ], acpos[G2_MAXQ];
     uint32_t store_prefix;        // strategy[id+1] == PREFIX (byte model only)
+    uint32_t pre_is_prev;         // the pre-intersection equals position id-1's candidate set
 };
 
 struct GmProgram {
