@@ -113,7 +113,7 @@ void g2_destroy(g2_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
-                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order})
+                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -145,6 +145,15 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     uint64_t md = 0;
     for (uint64_t v = 0; v < nv; v++) md = std::max(md, rp[v + 1] - rp[v]);
     c->max_degree = md;
+    // whole-graph view: every vertex is a source, in id order
+    c->sources_all = (ns == nv);
+    for (uint64_t i = 0; c->sources_all && i < ns; i++) c->sources_all = (src[i] == i);
+    {
+        std::vector<uint32_t> dg(nv + 1, 0);
+        for (uint64_t v = 0; v < nv; v++) dg[v] = (uint32_t)std::min<uint64_t>(rp[v + 1] - rp[v], 0xffffffffull);
+        if ((rc = ensure(c, c->deg, (nv + 1) * 4, "degrees"))) return rc;
+        if ((e = hipMemcpy(c->deg.ptr, dg.data(), (nv + 1) * 4, hipMemcpyHostToDevice))) return hip_err(c, e, "H2D degrees");
+    }
     // level-1 windows of the matching tasks: source i covers windows
     // [off[i], off[i+1]) of gm_chunk row entries each (one window when 0)
     c->gm_task_chunk = c->tune.gm_chunk;
@@ -181,6 +190,13 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     return mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);
+}
+
+int g2_plan_local_eligible(const g2_plan* p) {
+    if (!p) return G2_EINVAL;
+    g2_ctx tmp;  // host-side plan analysis only: no device state is touched
+    if (gm_compile(&tmp, p) != G2_OK) return G2_EINVAL;
+    return tmp.local_ok ? 1 : 0;
 }
 
 int g2_gm_set_plan(g2_ctx* c, const g2_plan* p) {

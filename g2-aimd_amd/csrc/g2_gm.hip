@@ -408,11 +408,16 @@ int gm_compile(g2_ctx* c, const g2_plan* p) {
     c->prog = g;
     c->plan = *p;
     c->has_plan = true;
+    gm_local_compile(c, p);
     return G2_OK;
 }
 
 int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
+    // whole-graph totals of a root-dominated plan: oriented local bitmaps
+    // (per-root counts and the byte model follow the reference's traversal)
+    if (c->local_ok && c->sources_all && !per_root && !account && !(flags & G2_RUN_CANONICAL))
+        return gm_local_run(c, count, st);
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);

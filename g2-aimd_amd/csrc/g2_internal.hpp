@@ -38,6 +38,17 @@ struct GmProgram {
     GmStep step[G2_MAXQ];
 };
 
+// Oriented local-bitmap form of a plan whose root is adjacent to, and ordered
+// below, every other query vertex (all k-cliques among the presets).  Every
+// table is by position >= 1; bn_mask excludes the root.
+struct LocalProgram {
+    uint32_t n;
+    uint32_t bn_mask[G2_MAXQ];     // back neighbours (positions >= 1) as a bit mask
+    uint32_t gt_mask[G2_MAXQ];     // positions j >= 1 with M(u_i) > M(u_j)
+    uint32_t lt_mask[G2_MAXQ];     // positions j >= 1 with M(u_i) < M(u_j)
+    uint32_t ne_mask[G2_MAXQ];     // positions j >= 1 with M(u_i) != M(u_j)
+};
+
 }  // namespace g2
 
 struct g2_ctx {
@@ -54,6 +65,8 @@ struct g2_ctx {
     g2::DevBuf rp, cols, src;
     uint64_t nv = 0, ne = 0, ns = 0;
     uint64_t max_degree = 0;
+    bool sources_all = false;     // sources == [0, nv) in order (whole-graph view)
+    g2::DevBuf deg;               // u32 degree of every vertex
     std::vector<uint64_t> h_gm_task_off;  // host copy: level-1 window offsets per source
     g2::DevBuf gm_task_off;
     uint32_t gm_task_chunk = 0;
@@ -70,6 +83,9 @@ struct g2_ctx {
     // matching
     bool has_plan = false;
     g2::GmProgram prog{};
+    bool local_ok = false;        // the plan qualifies for the oriented local-bitmap path
+    g2::LocalProgram lprog{};
+    g2::DevBuf big_list2;
     g2_plan plan{};
 };
 
@@ -84,5 +100,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
             uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st);
 int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_count, g2_stats* st);
 int gm_compile(g2_ctx* c, const g2_plan* p);
+void gm_local_compile(g2_ctx* c, const g2_plan* p);
+int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st);
 
 }  // namespace g2

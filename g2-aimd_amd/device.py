@@ -21,11 +21,12 @@ u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 G2_RUN_ACCOUNT = 1
+G2_RUN_CANONICAL = 2
 
 # every symbol include/g2hip.h declares
 EXPORTS = ("g2_create", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
            "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
-           "g2_pinned_free")
+           "g2_pinned_free", "g2_plan_local_eligible")
 
 
 class Tuning(C.Structure):
@@ -72,8 +73,17 @@ def hip_lib() -> C.CDLL:
     L.g2_gm_run.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64), vp, C.POINTER(Stats)]
     L.g2_device_info.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.c_char_p, C.c_uint32]
+    L.g2_plan_local_eligible.argtypes = [C.POINTER(G2Plan)]
     _hip = L
     return L
+
+
+def plan_local_eligible(plan) -> bool:
+    """Host-only check: is the plan counted by the oriented local-bitmap kernel?"""
+    rc = hip_lib().g2_plan_local_eligible(C.byref(plan))
+    if rc < 0:
+        raise RuntimeError("invalid plan")
+    return rc == 1
 
 
 def device_info(device: int = 0) -> dict:
@@ -156,11 +166,14 @@ class Executor:
     def set_plan(self, plan) -> None:
         self._check(hip_lib().g2_gm_set_plan(self._ctx, C.byref(plan)), "g2_gm_set_plan")
 
-    def run_gm(self, per_root: bool = False, account: bool = False) -> dict:
+    def run_gm(self, per_root: bool = False, account: bool = False, canonical: bool = False) -> dict:
+        """Match count; whole-graph totals of clique-like plans use the oriented
+        local-bitmap kernel unless `canonical` (or per-root / byte accounting)."""
         cnt = C.c_uint64()
         prc = np.zeros(max(self.ns, 1), np.uint64) if per_root else None
         st = Stats()
-        rc = hip_lib().g2_gm_run(self._ctx, G2_RUN_ACCOUNT if account else 0, C.byref(cnt),
+        flags = (G2_RUN_ACCOUNT if account else 0) | (G2_RUN_CANONICAL if canonical else 0)
+        rc = hip_lib().g2_gm_run(self._ctx, flags, C.byref(cnt),
                                  prc.ctypes.data if per_root else None, C.byref(st))
         self._check(rc, "g2_gm_run")
         out = {"count": cnt.value, "stats": st.to_dict()}

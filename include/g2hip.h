@@ -70,6 +70,9 @@ typedef struct g2_sched {
 } g2_sched;
 
 #define G2_RUN_ACCOUNT  1u        /* also compute the modelled algorithmic bytes */
+#define G2_RUN_CANONICAL 2u       /* matching: always walk the reference's traversal order
+                                     (otherwise whole-graph totals of clique-like plans use
+                                     the degree-oriented local-bitmap kernel) */
 
 int  g2_create(g2_ctx** ctx, int device, uint64_t mem_budget_bytes, const g2_tuning* tuning);
 void g2_destroy(g2_ctx* ctx);
@@ -97,6 +100,13 @@ int  g2_gm_set_plan(g2_ctx* ctx, const g2_plan* plan);
  * overflow-free count of app_gmatch/gmatch.h).  per_root_count nullable. */
 int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_count,
                g2_stats* stats);
+
+/* 1 if g2_gm_run counts this plan's whole-graph totals with the oriented
+ * local-bitmap kernel: the root is a back neighbour of, and ordered below,
+ * every other position, and the order conditions pick exactly one member of
+ * every automorphism class for every ranking (so the total does not depend on
+ * the vertex order).  0 otherwise; < 0 for an invalid plan.  Host only. */
+int  g2_plan_local_eligible(const g2_plan* plan);
 
 /* Device properties used for launch sizing */
 int  g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* name, uint32_t name_len);

@@ -78,7 +78,7 @@ def test_windows_and_views(g2, orc, rmat12):
     _, qadj = g2.query_preset(24)
     whole = orc.gm(rmat12, orc.OrcPlan.from_buffer_copy(bytes(g2.plan_build(qadj))))["count"]
     with g2.Executor(device=0, gm_chunk=16) as ex:
-        both(g2, orc, ex, rmat12, qadj)
+        both(g2, orc, ex, rmat12, qadj, account=True)
         tot = 0
         bins = np.arange(rmat12.nv) % 4
         for b in range(4):
@@ -86,3 +86,37 @@ def test_windows_and_views(g2, orc, rmat12):
             view = g2.view_materialize(rmat12, src, 2, 3)
             tot += both(g2, orc, ex, view, qadj, sources=src)["count"]
         assert tot == whole
+
+
+CLIQUES = [0, 23, 24, 25, 26, 27]
+
+
+def test_local_path_cliques(g2, orc, gpu_exec):
+    """Whole-graph clique totals through the oriented local-bitmap kernel
+    equal the oracle's (reference id order) and the canonical kernel's."""
+    print("case: local path cliques", flush=True)
+    graphs = [g2.gen_rmat(13, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess(),
+              G.gnp(g2, 400, 0.1, 3), G.complete(g2, 40),
+              g2.gen_chung_lu(20000, 2.3, 2.0, 800.0, 120000, 9).preprocess()]
+    for g in graphs:
+        gpu_exec.transfer(g)
+        for q in CLIQUES:
+            p = g2.plan_for_preset(q)
+            gpu_exec.set_plan(p)
+            fast = gpu_exec.run_gm()
+            canon = gpu_exec.run_gm(canonical=True)
+            ref = orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
+            assert fast["count"] == ref == canon["count"], (q, fast["count"], ref, canon["count"])
+
+
+def test_local_path_large_root_pass(g2, orc):
+    print("case: local path large roots", flush=True)
+    g = g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
+    with g2.Executor(device=0, wave_scratch_words=200, big_waves=32) as ex:
+        ex.transfer(g)
+        for q in (0, 24, 25):
+            p = g2.plan_for_preset(q)
+            ex.set_plan(p)
+            got = ex.run_gm()
+            assert got["stats"]["big_roots"] > 0
+            assert got["count"] == orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
