@@ -72,6 +72,8 @@ def model_bytes(g2, ex, g, plan, total, frac, seed=11):
     bytes add up, so with frac < 1 a seeded uniform root sample is accounted
     and scaled by nv/|sample| (Horvitz-Thompson); the same scaling of the
     sample's matches is reported next to the exact total as a check."""
+    if frac <= 0:
+        return None, "not computed (--model-sample 0)"
     if frac >= 1.0:
         ex.set_plan(plan)
         r = ex.run_gm(account=True)
@@ -85,12 +87,16 @@ def model_bytes(g2, ex, g, plan, total, frac, seed=11):
         mx.transfer(g, src)
         mx.set_plan(plan)
         r = mx.run_gm(account=True)
-    scale = g.nv / n
-    est_matches = r["count"] * scale
-    note = (f"estimated from a seeded uniform sample of {n} of {g.nv} roots "
-            f"(sample matches x nv/n = {est_matches:.4g} vs exact {total})")
+    # ratio estimator: modelled bytes per match of the sample x the exact
+    # match total (per-root work is heavy tailed; the plain nv/n scaling of
+    # the sample's matches is shown as the spread indicator)
+    est_matches = r["count"] * g.nv / n
+    per_match = r["stats"]["alg_bytes"] / max(r["count"], 1)
+    note = (f"ratio estimate: {per_match:.1f} modelled bytes/match on a seeded uniform sample "
+            f"of {n} of {g.nv} roots x {total} exact matches (sample matches x nv/n = "
+            f"{est_matches:.4g})")
     log("model bytes: " + note)
-    return int(r["stats"]["alg_bytes"] * scale), note
+    return int(per_match * total), note
 
 
 def cpu_baseline(orc, g, plan, target_s, seed=7):
@@ -122,9 +128,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--gm-chunk", type=int, default=0)
-    ap.add_argument("--model-sample", type=float, default=0.01,
-                    help="root fraction accounted for the byte model (1 = exact)")
+    ap.add_argument("--model-sample", type=float, default=0.02,
+                    help="root fraction accounted for the byte model (1 = exact, 0 = skip)")
+    ap.add_argument("--traffic-bytes", type=float, default=None,
+                    help="HBM bytes per launch of the dominant kernel from rocprof "
+                         "(profiles/), reported as roofline.traffic")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MB "
+                         "(rocprof counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration_mb:
+        import torch
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {n * 4} bytes 3 times")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the hot path shards by root ranges with one host-summed u64 per GPU, "
@@ -148,7 +170,10 @@ def main():
     cnt, ms_step, kernel_ms = time_gm(ex, args.steps, args.warmup)
     alg_bytes, sample_note = model_bytes(g2, ex, g, plan24, cnt, args.model_sample)
     value = cnt / (ms_step / 1e3)
-    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9 if alg_bytes else None
+    traffic = None
+    if args.traffic_bytes:
+        traffic = args.traffic_bytes
     out = {
         "metric": METRIC,
         "value": value,
@@ -170,7 +195,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+            "traffic_GBs": traffic / (kernel_ms / 1e3) / 1e9 if traffic else None,
+            "traffic_frac": traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
             "kernel": "gm_local_kernel<4>", "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes,
             "alg_bytes_source": sample_note,
         },
