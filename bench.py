@@ -99,6 +99,18 @@ def model_bytes(g2, ex, g, plan, total, frac, seed=11):
     return int(per_match * total), note
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) recorded in
+    profiles/<name> by the rocprofv3 passes listed there; None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(orc, g, plan, target_s, seed=7):
     """oracle on a seeded random root sample sized to ~target_s seconds"""
     threads = os.cpu_count() or 1
@@ -171,9 +183,9 @@ def main():
     alg_bytes, sample_note = model_bytes(g2, ex, g, plan24, cnt, args.model_sample)
     value = cnt / (ms_step / 1e3)
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9 if alg_bytes else None
-    traffic = None
-    if args.traffic_bytes:
-        traffic = args.traffic_bytes
+    traffic = args.traffic_bytes
+    if traffic is None and args.scale == 22:
+        traffic = profiled_traffic("gm_local_kernel_q24_rmat22.md")
     out = {
         "metric": METRIC,
         "value": value,
