@@ -176,6 +176,8 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
     if (!c) return G2_EINVAL;
     if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
     if (hist && hist_len < 2) return set_err(c, G2_EINVAL, "hist_len must be >= 2");
+    if (sched && sched->begin && sched->begin >= sched->end)
+        return set_err(c, G2_EINVAL, "empty schedule range");
     if (sched && (sched->begin > sched->end || sched->end > c->ns))
         return set_err(c, G2_EINVAL, "schedule range outside the sources");
     if (sched && sched->order) {

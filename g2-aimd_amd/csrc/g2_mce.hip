@@ -38,13 +38,15 @@ struct MceArgs {
     const uint64_t* __restrict__ rp;
     const uint32_t* __restrict__ cols;
     const uint32_t* __restrict__ src;
-    const uint32_t* __restrict__ order;  // task -> source index (nullable)
-    const uint32_t* __restrict__ list;   // task -> source index of the large-root pass (nullable)
+    // task codes: source index | part j << 32 | parts k << 48 (the task walks
+    // the root's first-level children c with c mod k == j); null = task t is
+    // source t, whole
+    const uint64_t* __restrict__ tasks;
     uint64_t first;                      // first task (range runs)
     uint64_t ntasks;                     // one past the last task
     uint64_t* scratch;
     uint64_t scratch_words;              // per wave
-    uint32_t* big_list;                  // deferred source indices (null in the large-root pass)
+    uint64_t* big_list;                  // deferred task codes (null in the large-root pass)
     unsigned long long* ctr;
     unsigned long long* hist;
     uint32_t hist_len;
@@ -83,23 +85,25 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         if (lane == 0) t = atomicAdd(&a.ctr[CTR_NEXT], 1ull);
         t = uniform((uint64_t)t) + a.first;
         if (t >= a.ntasks) break;
-        const uint32_t si = uniform(a.list ? a.list[t] : (a.order ? a.order[t] : (uint32_t)t));
+        const uint64_t code = uniform(a.tasks ? a.tasks[t] : (uint64_t)t);
+        const uint32_t si = (uint32_t)code;
+        const uint32_t part = (uint32_t)(code >> 32) & 0xffffu;
+        const uint32_t parts = (uint32_t)(code >> 48) ? (uint32_t)(code >> 48) : 1u;
         const uint32_t v = uniform(a.src[si]);
         const uint64_t lo = uniform(a.rp[v]);
         const uint64_t d = uniform(a.rp[v + 1]) - lo;
         const uint32_t* __restrict__ C = a.cols + lo;
         uint64_t root_nodes = 0;
 
-        if (d == 0) {  // no subgraph is generated for an isolated root
-            if (a.per_root && lane == 0) a.per_root[si] = 0;
-            continue;
-        }
+        if (d == 0) continue;  // no subgraph is generated for an isolated root
         const uint64_t s = uniform(lower_bound(C, 0, d, v));
         const uint64_t p0 = d - s;
         if (p0 == 0) {  // root node with empty P and non-empty X: one node, no clique
-            if (ACCOUNT && lane == 0) my_bytes += 5 * (1 + d);
-            my_nodes += 1;
-            if (a.per_root && lane == 0) a.per_root[si] = 1;
+            if (part == 0) {
+                if (ACCOUNT && lane == 0) my_bytes += 5 * (1 + d);
+                my_nodes += 1;
+                if (a.per_root && lane == 0) atomicAdd(&a.per_root[si], 1ull);
+            }
             continue;
         }
         const uint64_t W = (p0 + 63) / 64, WX = (s + 63) / 64, FW = 3 * W + WX;
@@ -112,7 +116,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
             if (lane == 0) {
                 if (a.big_list) {
                     const unsigned long long idx = atomicAdd(&a.ctr[CTR_NBIG], 1ull);
-                    a.big_list[idx] = si;
+                    a.big_list[idx] = code;
                     atomicMax(&a.ctr[CTR_MAXNEED], (unsigned long long)need);
                 } else {
                     atomicMax(&a.ctr[CTR_ERR], (unsigned long long)need);
@@ -178,12 +182,15 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         for (uint64_t i = lane; i < WX; i += kWave) frames[3 * W + i] = ones(s - i * 64);
         uint64_t depth = 0;
         bool fresh = true;
+        uint32_t child = 0;  // first-level children passed so far
         while (true) {
             uint64_t* F = frames + depth * FW;
             uint64_t* XX = F + 3 * W;
             bool descend = false;
+            // the root node belongs to part 0 of a split root
+            const bool own = depth > 0 || part == 0;
             if (fresh) {
-                root_nodes++;
+                if (own) root_nodes++;
                 const bool p_any = ballot(pw != 0) != 0;
                 uint64_t xx_or = 0;
                 for (uint64_t i = lane; i < WX; i += kWave) xx_or |= XX[i];
@@ -194,7 +201,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                     for (uint64_t i = lane; i < WX; i += kWave) cx += __popcll(XX[i]);
                     np = wave_sum(__popcll(pw));
                     nPX = np + wave_sum(__popcll(xpw)) + wave_sum(cx);
-                    if (lane == 0) my_bytes += 5 * (depth + 1 + nPX) * (depth ? 2 : 1);
+                    if (lane == 0 && own) my_bytes += 5 * (depth + 1 + nPX) * (depth ? 2 : 1);
                 }
                 if (!p_any) {
                     if (!x_any && lane == 0) {  // maximal clique R, |R| = depth + 1
@@ -220,7 +227,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                                 for (uint64_t i = 0; i < W; i++) sc += __popcll(row[i] & read_lane(pw, (uint32_t)i));
                                 const uint64_t key = (sc << 32) | k;
                                 best = key > best ? key : best;
-                                if (ACCOUNT) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
+                                if (ACCOUNT && own) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
                             }
                         }
                     }
@@ -238,7 +245,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                                 for (uint64_t i = 0; i < W; i++) sc += __popcll(row[i] & read_lane(pw, (uint32_t)i));
                                 const uint64_t key = (sc << 32) | k;
                                 best = key > best ? key : best;
-                                if (ACCOUNT) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
+                                if (ACCOUNT && own) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
                             }
                         }
                     }
@@ -246,7 +253,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                     const uint64_t pivot = best & 0xffffffffull;
                     const uint64_t* prow = adjP + pivot * W;
                     qw = lane < W ? (pw & ~prow[lane]) : 0;
-                    if (ACCOUNT) {
+                    if (ACCOUNT && own) {
                         if (lane == 0) my_bytes += cost_isect(np, degree_of(a.rp, C[pivot]));
                         uint64_t nz = ballot(qw != 0);
                         while (nz) {
@@ -273,6 +280,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                     pw &= ~m;
                     xpw |= m;
                 }
+                if (depth == 0 && (child++ % parts) != part) continue;  // another part's child
                 if (lane < W) {
                     F[lane] = pw;
                     F[W + lane] = xpw;
@@ -304,7 +312,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
             }
         }
         my_nodes += root_nodes;
-        if (a.per_root && lane == 0) a.per_root[si] = root_nodes;
+        if (a.per_root && lane == 0 && root_nodes) atomicAdd(&a.per_root[si], (unsigned long long)root_nodes);
     }
 
     // ---- flush per-wave and per-block tallies ----------------------------
@@ -330,16 +338,30 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     int rc;
     if ((rc = ensure(c, c->counters, CTR_N * 8, "counters"))) return rc;
     if ((rc = ensure(c, c->hist, (uint64_t)hl * 8, "histogram"))) return rc;
-    if ((rc = ensure(c, c->list, std::max<uint64_t>(c->ns, 1) * 4, "deferred roots"))) return rc;
     if ((rc = ensure(c, c->scratch, nwaves * sw * 8, "wave arenas"))) return rc;
     if (per_root_nodes && (rc = ensure(c, c->per_root, std::max<uint64_t>(c->ns, 1) * 8, "per-root counts"))) return rc;
-    const uint32_t* order = nullptr;
-    if (sched && sched->order) {
-        if ((rc = ensure(c, c->order, std::max<uint64_t>(c->ns, 1) * 4, "schedule"))) return rc;
-        hipError_t e = hipMemcpyAsync(c->order.ptr, sched->order, c->ns * 4, hipMemcpyHostToDevice, c->stream);
-        if (e != hipSuccess) return hip_err(c, e, "H2D schedule");
-        order = (const uint32_t*)c->order.ptr;
+    // task list: sources in schedule order, heavy ones dealt into parts
+    const uint64_t* tasks = nullptr;
+    uint64_t first = 0, ntasks = c->ns;
+    if (sched && (sched->order || sched->split || sched->end > sched->begin)) {
+        const uint64_t b = sched->end > sched->begin ? sched->begin : 0;
+        const uint64_t en = sched->end > sched->begin ? sched->end : c->ns;
+        std::vector<uint64_t> h;
+        h.reserve(en - b);
+        for (uint64_t t = b; t < en; t++) {
+            const uint64_t si = sched->order ? sched->order[t] : t;
+            const uint64_t k = sched->split ? std::max<uint32_t>(1, std::min<uint32_t>(sched->split[si], 0xffffu)) : 1;
+            for (uint64_t j = 0; j < k; j++) h.push_back(si | (j << 32) | (k << 48));
+        }
+        if ((rc = ensure(c, c->order, std::max<uint64_t>(h.size(), 1) * 8, "task list"))) return rc;
+        if (!h.empty()) {
+            hipError_t e = hipMemcpy(c->order.ptr, h.data(), h.size() * 8, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_err(c, e, "H2D task list");
+        }
+        tasks = (const uint64_t*)c->order.ptr;
+        ntasks = h.size();
     }
+    if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks, 1) * 8, "deferred roots"))) return rc;
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
     if ((e = hipMemsetAsync(ctr, 0, CTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
@@ -350,17 +372,12 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     a.rp = (const uint64_t*)c->rp.ptr;
     a.cols = (const uint32_t*)c->cols.ptr;
     a.src = (const uint32_t*)c->src.ptr;
-    a.order = order;
-    a.list = nullptr;
-    a.first = 0;
-    a.ntasks = c->ns;
-    if (sched && sched->end > sched->begin) {
-        a.first = sched->begin;
-        a.ntasks = sched->end;
-    }
+    a.tasks = tasks;
+    a.first = first;
+    a.ntasks = ntasks;
     a.scratch = (uint64_t*)c->scratch.ptr;
     a.scratch_words = sw;
-    a.big_list = (uint32_t*)c->list.ptr;
+    a.big_list = (uint64_t*)c->list.ptr;
     a.ctr = ctr;
     a.hist = (unsigned long long*)c->hist.ptr;
     a.hist_len = hl;
@@ -389,7 +406,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
         bwaves = (bwaves + bw - 1) / bw * bw;
         if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
-        a.list = (const uint32_t*)c->list.ptr;
+        a.tasks = (const uint64_t*)c->list.ptr;
         a.first = 0;
         a.ntasks = nbig;
         a.scratch = (uint64_t*)c->big_scratch.ptr;

@@ -46,7 +46,7 @@ class Stats(C.Structure):
 
 class Sched(C.Structure):
     _fields_ = [("order", C.c_void_p), ("n", C.c_uint64), ("begin", C.c_uint64),
-                ("end", C.c_uint64)]
+                ("end", C.c_uint64), ("split", C.c_void_p)]
 
 
 def hip_lib() -> C.CDLL:
@@ -139,20 +139,29 @@ class Executor:
                     "g2_upload_view")
         self.ns = len(src)
 
-    def run_mce(self, hist_len: int = 256, per_root: bool = False, order=None,
+    def run_mce(self, hist_len: int = 256, per_root: bool = False, order=None, split=None,
                 account: bool = False) -> dict:
-        """Maximal clique enumeration over the uploaded sources."""
+        """Maximal clique enumeration over the uploaded sources.  `order`
+        (permutation of source positions) and `split` (per source: parts its
+        first-level children are dealt into) only change the schedule."""
         total = C.c_uint64()
         hist = np.zeros(hist_len, np.uint64)
         prn = np.zeros(max(self.ns, 1), np.uint64) if per_root else None
         st = Stats()
         sched = None
-        if order is not None:
-            o = np.ascontiguousarray(order, dtype=np.uint32)
-            if len(o) != self.ns:
-                raise ValueError("order must be a permutation of the sources")
-            sched = Sched(o.ctypes.data, len(o), 0, 0)
-            self._order_keep = o
+        if order is not None or split is not None:
+            o = sp = None
+            if order is not None:
+                o = np.ascontiguousarray(order, dtype=np.uint32)
+                if len(o) != self.ns:
+                    raise ValueError("order must be a permutation of the sources")
+            if split is not None:
+                sp = np.ascontiguousarray(split, dtype=np.uint32)
+                if len(sp) != self.ns:
+                    raise ValueError("split needs one entry per source")
+            sched = Sched(o.ctypes.data if o is not None else None, self.ns, 0, 0,
+                          sp.ctypes.data if sp is not None else None)
+            self._sched_keep = (o, sp)
         rc = hip_lib().g2_mce_run(self._ctx, C.byref(sched) if sched else None,
                                   G2_RUN_ACCOUNT if account else 0, C.byref(total),
                                   hist.ctypes.data, hist_len,

@@ -113,7 +113,7 @@ RunResult Executor::run_bk(uint32_t hist_len) {
     g2h_aimd_init(&aimd);
     uint64_t total = 0;
     for (uint64_t i = 0; i < ns_;) {
-        g2_sched s{nullptr, ns_, i, std::min<uint64_t>(ns_, i + aimd.chunk)};
+        g2_sched s{nullptr, ns_, i, std::min<uint64_t>(ns_, i + aimd.chunk), nullptr};
         uint64_t cl = 0;
         const int rc = g2_mce_run(ctx_, &s, 0, &cl, h.data(), hist_len, nullptr, nullptr);
         bool overflow = false;

@@ -62,11 +62,14 @@ typedef struct g2_stats {
     uint64_t max_scratch_words;   /* largest per-root scratch need seen */
 } g2_stats;
 
-/* root schedule for g2_mce_run (NULL = the given source order) */
+/* root schedule for g2_mce_run (NULL = the given source order, one wave per root) */
 typedef struct g2_sched {
-    const uint32_t* order;        /* permutation of [0, ns): processing order of the sources */
-    uint64_t        n;            /* must equal the uploaded source count */
-    uint64_t        begin, end;   /* process tasks [begin, end) of that order only (0,0 = all) */
+    const uint32_t* order;        /* permutation of [0, ns): processing order of the sources (NULL = as given) */
+    uint64_t        n;            /* must equal the uploaded source count when order is set */
+    uint64_t        begin, end;   /* process positions [begin, end) of that order only (0,0 = all) */
+    const uint32_t* split;        /* per source (source order): deal the root's first-level
+                                     children into this many tasks, child c going to task
+                                     c mod k (NULL or 0/1 = whole root; at most 65535) */
 } g2_sched;
 
 #define G2_RUN_ACCOUNT  1u        /* also compute the modelled algorithmic bytes */

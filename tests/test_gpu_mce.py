@@ -94,6 +94,31 @@ def test_large_root_pass_and_schedule(g2, orc):
         assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
 
 
+def test_split_roots(g2, orc, gpu_exec, cfg1):
+    """Dealing each root's first-level children over several waves (the
+    heavy-root split of the learned scheduler) never changes a result."""
+    print("case: split roots", flush=True)
+    g, _ = cfg1
+    ref = orc.mce(g, per_root=True, hist_len=256)
+    gpu_exec.transfer(g)
+    rng = np.random.default_rng(5)
+    split = rng.integers(1, 9, size=g.nv).astype(np.uint32)
+    order = rng.permutation(g.nv).astype(np.uint32)
+    for kw in ({"split": split}, {"split": split, "order": order}, {"split": np.full(g.nv, 64, np.uint32)}):
+        got = gpu_exec.run_mce(per_root=True, **kw)
+        assert got["total"] == ref["total"]
+        assert np.array_equal(got["hist"], ref["hist"])
+        assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+    acc = gpu_exec.run_mce(account=True, split=split)
+    assert acc["stats"]["alg_bytes"] == ref["bytes"]
+    dense = G.gnp(g2, 200, 0.5, 11)
+    ref = orc.mce(dense, per_root=True)
+    gpu_exec.transfer(dense)
+    got = gpu_exec.run_mce(per_root=True, split=np.full(dense.nv, 7, np.uint32))
+    assert got["total"] == ref["total"]
+    assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+
+
 def test_rmat_mce(g2, orc, gpu_exec):
     """R-MAT hubs: neighbourhoods of thousands of vertices (large-root pass)."""
     print("case: rmat-12 mce", flush=True)
