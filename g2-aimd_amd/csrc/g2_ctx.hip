@@ -113,7 +113,7 @@ void g2_destroy(g2_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
-                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2})
+                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);

@@ -79,6 +79,7 @@ struct g2_ctx {
     g2::DevBuf hist;
     g2::DevBuf per_root;
     g2::DevBuf order;
+    g2::DevBuf spans;
 
     // matching
     bool has_plan = false;

@@ -1,0 +1,553 @@
+// csrc/g2_learn.hip -- root features and the cost model of the learned root
+// scheduler (no counterpart in the reference, whose roots run in source order).
+//
+// * feature_kernel: one wave64 per root computes integer features
+//     deg(v), 2-path count sum_{u in N(v)} (deg(u)-1), triangles t(v),
+//     neighbours sharing >= 1 common neighbour with v, and p0 = |{u > v}|
+//   (the size of the root's P set in the maximal-clique search tree);
+//   triangles are counted by streaming the shorter of N(v), N(u) and binary
+//   searching the other, one neighbour u at a time.
+// * mlp: 8 -> 64 -> 64 -> 1 with ReLU, BF16 weights and activations on the
+//   matrix cores (v_mfma_f32_16x16x16bf16_1k), FP32 accumulation, FP32 master
+//   weights and Adam.  A wave owns 16 samples: layer 1 is 4 MFMA tiles (K padded
+//   8 -> 16), layer 2 is 4x4 tiles, layer 3 a 16-lane reduction.  The backward
+//   pass forms dW2 = h1^T dh2, dW1 = x^T dh1 and dh1 = dh2 W2^T on the same
+//   instruction and accumulates each wave's gradient in registers before one
+//   atomic flush.
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "g2_device.hpp"
+#include "g2_internal.hpp"
+
+namespace g2 {
+
+// ---------------------------------------------------------------------------
+// features
+// ---------------------------------------------------------------------------
+struct FeatArgs {
+    const uint64_t* __restrict__ rp;
+    const uint32_t* __restrict__ cols;
+    const uint32_t* __restrict__ src;
+    uint64_t ns;
+    unsigned long long* next;
+    int64_t* out;  // ns x G2_NFEAT_INT
+};
+
+__global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
+    const uint32_t lane = lane_id();
+    while (true) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(a.next, 1ull);
+        t = uniform((uint64_t)t);
+        if (t >= a.ns) break;
+        const uint32_t v = uniform(a.src[t]);
+        const uint64_t lo = a.rp[v], hi = a.rp[v + 1], d = hi - lo;
+        uint64_t two = 0, p0 = 0, tri2 = 0, shared = 0;
+        for (uint64_t e = lo + lane; e < hi; e += kWave) {
+            const uint32_t u = a.cols[e];
+            two += (a.rp[u + 1] - a.rp[u]) - 1;
+            p0 += u > v;
+        }
+        for (uint64_t e = lo; e < hi; e++) {
+            const uint32_t u = a.cols[e];
+            const uint64_t ulo = a.rp[u], uhi = a.rp[u + 1];
+            uint64_t c = 0;
+            if (uhi - ulo <= d) {
+                for (uint64_t k = ulo + lane; k < uhi; k += kWave) c += contains(a.cols, lo, hi, a.cols[k]);
+            } else {
+                for (uint64_t k = lo + lane; k < hi; k += kWave) c += contains(a.cols, ulo, uhi, a.cols[k]);
+            }
+            c = wave_sum(c);
+            tri2 += c;
+            shared += c > 0;
+        }
+        two = wave_sum(two);
+        p0 = wave_sum(p0);
+        if (lane == 0) {
+            int64_t* o = a.out + t * G2_NFEAT_INT;
+            o[0] = (int64_t)d;
+            o[1] = (int64_t)two;
+            o[2] = (int64_t)(tri2 / 2);
+            o[3] = (int64_t)shared;
+            o[4] = (int64_t)p0;
+        }
+    }
+}
+
+int features_run(g2_ctx* c, int64_t* out) {
+    int rc;
+    if ((rc = ensure(c, c->counters, 8, "counters"))) return rc;
+    DevBuf dout;
+    if ((rc = ensure(c, dout, std::max<uint64_t>(c->ns, 1) * G2_NFEAT_INT * 8, "features"))) return rc;
+    hipError_t e;
+    if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
+    FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
+               (unsigned long long*)c->counters.ptr, (int64_t*)dout.ptr};
+    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / 4);
+    hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
+    if ((e = hipGetLastError())) return hip_err(c, e, "feature launch");
+    if (c->ns && (e = hipMemcpyAsync(out, dout.ptr, c->ns * G2_NFEAT_INT * 8, hipMemcpyDeviceToHost, c->stream)))
+        return hip_err(c, e, "D2H features");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "feature sync");
+    (void)hipFree(dout.ptr);
+    c->used -= dout.bytes;
+    return G2_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MLP 8 -> 64 -> 64 -> 1 on bf16 MFMA
+// ---------------------------------------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = 8, kH = 64;
+// parameter offsets (fp32 master copy)
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH, kNParam = oB3 + 1;
+static_assert(kNParam == G2_MLP_NPARAM, "parameter count");
+
+// float -> bf16, round to nearest even (NaN-free inputs)
+__device__ __host__ inline uint16_t f2bf(float f) {
+    uint32_t x;
+    __builtin_memcpy(&x, &f, 4);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (uint16_t)(x >> 16);
+}
+__device__ __host__ inline float bf2f(uint16_t b) {
+    uint32_t x = (uint32_t)b << 16;
+    float f;
+    __builtin_memcpy(&f, &x, 4);
+    return f;
+}
+
+struct MlpShared {
+    uint16_t w1[16][kH];     // rows >= kIn are zero (K padded to 16)
+    uint16_t w2[kH][kH];
+    float w3[kH];            // bf16-rounded, kept as fp32 for the VALU dot
+    float b1[kH], b2[kH], b3;
+};
+
+__device__ inline void load_weights(MlpShared& S, const float* __restrict__ p) {
+    for (int i = threadIdx.x; i < 16 * kH; i += blockDim.x) {
+        const int k = i / kH, n = i % kH;
+        S.w1[k][n] = k < kIn ? f2bf(p[oW1 + k * kH + n]) : 0;
+    }
+    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) S.w2[i / kH][i % kH] = f2bf(p[oW2 + i]);
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+        S.w3[i] = bf2f(f2bf(p[oW3 + i]));
+        S.b1[i] = p[oB1 + i];
+        S.b2[i] = p[oB2 + i];
+    }
+    if (threadIdx.x == 0) S.b3 = p[oB3];
+    __syncthreads();
+}
+
+struct WaveTile {  // per-wave LDS for one tile of 16 samples
+    uint16_t x[16][16];      // bf16 inputs, K padded
+    uint16_t h1[16][kH];     // bf16 post-ReLU layer-1 activations
+    uint16_t g[16][kH];      // bf16 gradient staging (dh2, then dh1)
+    float y[16];
+};
+
+__device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// forward of one 16-sample tile; leaves h1 (bf16) in T and h2 (fp32, D layout:
+// lane holds rows 4*(lane/16)+i, columns 16*t + lane%16) in h2; returns y in T.y
+__device__ inline void forward_tile(const MlpShared& S, WaveTile& T, f32x4 h2[4]) {
+    const uint32_t lane = lane_id();
+    const uint32_t r = lane % 16, q = lane / 16;
+    // layer 1: [16 x 16] x [16 x 64]
+    bf16x4 ax;
+    for (int i = 0; i < 4; i++) ax[i] = (short)T.x[r][4 * q + i];
+    f32x4 h1[4];
+    for (int t = 0; t < 4; t++) {
+        bf16x4 b;
+        for (int i = 0; i < 4; i++) b[i] = (short)S.w1[4 * q + i][16 * t + r];
+        h1[t] = mfma16(ax, b, f32x4{0, 0, 0, 0});
+    }
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            const float z = h1[t][i] + S.b1[16 * t + r];
+            T.h1[4 * q + i][16 * t + r] = f2bf(z > 0 ? z : 0);
+        }
+    wave_sync();
+    // layer 2: [16 x 64] x [64 x 64]
+    for (int t = 0; t < 4; t++) h2[t] = f32x4{0, 0, 0, 0};
+    for (int s = 0; s < 4; s++) {
+        bf16x4 a;
+        for (int i = 0; i < 4; i++) a[i] = (short)T.h1[r][16 * s + 4 * q + i];
+        for (int t = 0; t < 4; t++) {
+            bf16x4 b;
+            for (int i = 0; i < 4; i++) b[i] = (short)S.w2[16 * s + 4 * q + i][16 * t + r];
+            h2[t] = mfma16(a, b, h2[t]);
+        }
+    }
+    // bias, ReLU, layer 3 partial dots
+    float part[4] = {0, 0, 0, 0};
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            const float z = h2[t][i] + S.b2[16 * t + r];
+            h2[t][i] = z > 0 ? z : 0;
+            part[i] += h2[t][i] * S.w3[16 * t + r];
+        }
+    for (int i = 0; i < 4; i++) {
+        float v = part[i];
+        for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
+        if (r == 0) T.y[4 * q + i] = v + S.b3;
+    }
+    wave_sync();
+}
+
+struct MlpArgs {
+    const float* __restrict__ params;   // fp32 master
+    const float* __restrict__ x;        // n x 8
+    const float* __restrict__ yt;       // n targets (train)
+    const uint32_t* __restrict__ idx;   // batch -> sample (nullable: identity)
+    uint64_t n;                         // samples in this launch
+    float inv_n;                        // 1 / batch size (MSE mean)
+    float* grad;                        // G2_MLP_NPARAM accumulators (train)
+    float* loss;                        // sum of squared errors (train)
+    float* yout;                        // predictions (infer)
+};
+
+__device__ inline void load_tile(const MlpArgs& a, WaveTile& T, uint64_t base) {
+    const uint32_t lane = lane_id();
+    for (uint32_t k = lane; k < 16 * 16; k += kWave) {
+        const uint32_t m = k / 16, f = k % 16;
+        const uint64_t s = base + m;
+        float v = 0;
+        if (f < kIn && s < a.n) v = a.x[(a.idx ? a.idx[s] : s) * kIn + f];
+        T.x[m][f] = f2bf(v);
+    }
+    wave_sync();
+}
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(MlpArgs a) {
+    __shared__ MlpShared S;
+    __shared__ WaveTile tiles[4];
+    load_weights(S, a.params);
+    WaveTile& T = tiles[threadIdx.x / kWave];
+    const uint32_t lane = lane_id();
+    const uint64_t nt = (a.n + 15) / 16;
+    for (uint64_t tile = (uint64_t)blockIdx.x * 4 + threadIdx.x / kWave; tile < nt; tile += (uint64_t)gridDim.x * 4) {
+        load_tile(a, T, tile * 16);
+        f32x4 h2[4];
+        forward_tile(S, T, h2);
+        if (lane < 16 && tile * 16 + lane < a.n) a.yout[tile * 16 + lane] = T.y[lane];
+        wave_sync();
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_train_kernel(MlpArgs a) {
+    __shared__ MlpShared S;
+    __shared__ WaveTile tiles[4];
+    load_weights(S, a.params);
+    WaveTile& T = tiles[threadIdx.x / kWave];
+    const uint32_t lane = lane_id();
+    const uint32_t r = lane % 16, q = lane / 16;
+    const uint64_t nt = (a.n + 15) / 16;
+    // per-wave gradient accumulators in MFMA D layout
+    f32x4 gw2[4][4], gw1[4];
+    for (int p = 0; p < 4; p++) {
+        gw1[p] = f32x4{0, 0, 0, 0};
+        for (int t = 0; t < 4; t++) gw2[p][t] = f32x4{0, 0, 0, 0};
+    }
+    float gw3[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb1[4] = {0, 0, 0, 0}, gb3 = 0, sse = 0;
+    for (uint64_t tile = (uint64_t)blockIdx.x * 4 + threadIdx.x / kWave; tile < nt; tile += (uint64_t)gridDim.x * 4) {
+        load_tile(a, T, tile * 16);
+        f32x4 h2[4];
+        forward_tile(S, T, h2);
+        // dy for my 4 rows
+        float dy[4];
+        for (int i = 0; i < 4; i++) {
+            const uint64_t s = tile * 16 + 4 * q + i;
+            float e = 0;
+            if (s < a.n) e = T.y[4 * q + i] - a.yt[a.idx ? a.idx[s] : s];
+            if (r == 0) sse += e * e;
+            dy[i] = 2.0f * e * a.inv_n;
+            if (r == 0) gb3 += dy[i];
+        }
+        // dh2 = dy * w3 masked by ReLU; dW3 += h2^T dy; db2
+        for (int t = 0; t < 4; t++) {
+            const float w3 = S.w3[16 * t + r];
+            float col3 = 0, colb = 0;
+            for (int i = 0; i < 4; i++) {
+                const float g = h2[t][i] > 0 ? dy[i] * w3 : 0.f;
+                col3 += h2[t][i] * dy[i];
+                colb += g;
+                T.g[4 * q + i][16 * t + r] = f2bf(g);
+            }
+            gw3[t] += col3;
+            gb2[t] += colb;
+        }
+        wave_sync();
+        // dW2[k][n] += sum_m h1[m][k] dh2[m][n]   (M = k, K = samples, N = n)
+        for (int p = 0; p < 4; p++) {
+            bf16x4 aa;
+            for (int i = 0; i < 4; i++) aa[i] = (short)T.h1[4 * q + i][16 * p + r];
+            for (int t = 0; t < 4; t++) {
+                bf16x4 b;
+                for (int i = 0; i < 4; i++) b[i] = (short)T.g[4 * q + i][16 * t + r];
+                gw2[p][t] = mfma16(aa, b, gw2[p][t]);
+            }
+        }
+        // dh1 = dh2 W2^T masked by ReLU   (M = samples, K = n, N = k)
+        f32x4 d1[4];
+        for (int t = 0; t < 4; t++) d1[t] = f32x4{0, 0, 0, 0};
+        for (int s = 0; s < 4; s++) {
+            bf16x4 aa;
+            for (int i = 0; i < 4; i++) aa[i] = (short)T.g[r][16 * s + 4 * q + i];
+            for (int t = 0; t < 4; t++) {
+                bf16x4 b;
+                for (int i = 0; i < 4; i++) b[i] = (short)S.w2[16 * t + r][16 * s + 4 * q + i];
+                d1[t] = mfma16(aa, b, d1[t]);
+            }
+        }
+        wave_sync();
+        for (int t = 0; t < 4; t++) {
+            float colb = 0;
+            for (int i = 0; i < 4; i++) {
+                const float h = bf2f(T.h1[4 * q + i][16 * t + r]);
+                const float g = h > 0 ? d1[t][i] : 0.f;
+                colb += g;
+                T.g[4 * q + i][16 * t + r] = f2bf(g);
+            }
+            gb1[t] += colb;
+        }
+        wave_sync();
+        // dW1[k][n] += sum_m x[m][k] dh1[m][n]   (M = k padded to 16, K = samples, N = n)
+        {
+            bf16x4 aa;
+            for (int i = 0; i < 4; i++) aa[i] = (short)T.x[4 * q + i][r];
+            for (int t = 0; t < 4; t++) {
+                bf16x4 b;
+                for (int i = 0; i < 4; i++) b[i] = (short)T.g[4 * q + i][16 * t + r];
+                gw1[t] = mfma16(aa, b, gw1[t]);
+            }
+        }
+        wave_sync();
+    }
+    // flush: reduce the column sums over the 4 row groups, then one atomic each
+    for (int t = 0; t < 4; t++) {
+        float v3 = gw3[t], vb2 = gb2[t], vb1 = gb1[t];
+        v3 += __shfl_xor(v3, 16, 64);
+        v3 += __shfl_xor(v3, 32, 64);
+        vb2 += __shfl_xor(vb2, 16, 64);
+        vb2 += __shfl_xor(vb2, 32, 64);
+        vb1 += __shfl_xor(vb1, 16, 64);
+        vb1 += __shfl_xor(vb1, 32, 64);
+        if (q == 0) {
+            atomicAdd(&a.grad[oW3 + 16 * t + r], v3);
+            atomicAdd(&a.grad[oB2 + 16 * t + r], vb2);
+            atomicAdd(&a.grad[oB1 + 16 * t + r], vb1);
+        }
+    }
+    for (int p = 0; p < 4; p++)
+        for (int t = 0; t < 4; t++)
+            for (int i = 0; i < 4; i++) atomicAdd(&a.grad[oW2 + (16 * p + 4 * q + i) * kH + 16 * t + r], gw2[p][t][i]);
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            const int k = 4 * q + i;
+            if (k < kIn) atomicAdd(&a.grad[oW1 + k * kH + 16 * t + r], gw1[t][i]);
+        }
+    for (int m = 32; m >= 1; m >>= 1) {  // lanes with r == 0 hold the partial sums
+        gb3 += __shfl_xor(gb3, m, 64);
+        sse += __shfl_xor(sse, m, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&a.grad[oB3], gb3);
+        atomicAdd(a.loss, sse);
+    }
+}
+
+struct AdamArgs {
+    float* params;
+    const float* grad;
+    float* m;
+    float* v;
+    float lr, b1, b2, eps, c1, c2;  // c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
+};
+
+__global__ void adam_kernel(AdamArgs a) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNParam; i += gridDim.x * blockDim.x) {
+        const float g = a.grad[i];
+        const float m = a.b1 * a.m[i] + (1 - a.b1) * g;
+        const float v = a.b2 * a.v[i] + (1 - a.b2) * g * g;
+        a.m[i] = m;
+        a.v[i] = v;
+        a.params[i] -= a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
+    }
+}
+
+}  // namespace g2
+
+struct g2_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
+    float *x = nullptr, *y = nullptr, *out = nullptr;
+    uint32_t* idx = nullptr;
+    uint64_t cap = 0;
+    uint64_t step = 0;
+    uint32_t num_cus = 256;
+    std::mt19937_64 rng;
+};
+
+using namespace g2;
+
+static int mlp_ensure(g2_mlp* M, uint64_t n) {
+    if (n <= M->cap) return G2_OK;
+    for (void* p : {(void*)M->x, (void*)M->y, (void*)M->out, (void*)M->idx})
+        if (p) (void)hipFree(p);
+    if (hipMalloc(&M->x, n * kIn * 4) || hipMalloc(&M->y, n * 4) || hipMalloc(&M->out, n * 4) ||
+        hipMalloc(&M->idx, n * 4))
+        return G2_EHIP;
+    M->cap = n;
+    return G2_OK;
+}
+
+extern "C" {
+
+int g2_features(g2_ctx* c, int64_t* out) {
+    if (!c || !out) return G2_EINVAL;
+    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    return features_run(c, out);
+}
+
+int g2_mlp_create(g2_mlp** out, int device, uint64_t seed) {
+    if (!out) return G2_EINVAL;
+    auto* M = new g2_mlp;
+    M->device = device;
+    M->rng.seed(seed);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete M;
+        return G2_EHIP;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) == hipSuccess) M->num_cus = p.multiProcessorCount;
+    if (hipMalloc(&M->params, kNParam * 4) || hipMalloc(&M->grad, kNParam * 4) || hipMalloc(&M->m, kNParam * 4) ||
+        hipMalloc(&M->v, kNParam * 4) || hipMalloc(&M->loss, 4)) {
+        g2_mlp_destroy(M);
+        return G2_EHIP;
+    }
+    // He-uniform initialisation from the seed, zero biases
+    std::vector<float> h(kNParam, 0.f);
+    auto fill = [&](int off, int fan_in, int count) {
+        const float b = std::sqrt(6.0f / fan_in);
+        std::uniform_real_distribution<float> U(-b, b);
+        for (int i = 0; i < count; i++) h[off + i] = U(M->rng);
+    };
+    fill(oW1, kIn, kIn * kH);
+    fill(oW2, kH, kH * kH);
+    fill(oW3, kH, kH);
+    if (hipMemcpy(M->params, h.data(), kNParam * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(M->m, 0, kNParam * 4) != hipSuccess || hipMemset(M->v, 0, kNParam * 4) != hipSuccess) {
+        g2_mlp_destroy(M);
+        return G2_EHIP;
+    }
+    *out = M;
+    return G2_OK;
+}
+
+void g2_mlp_destroy(g2_mlp* M) {
+    if (!M) return;
+    (void)hipSetDevice(M->device);
+    for (void* p : {(void*)M->params, (void*)M->grad, (void*)M->m, (void*)M->v, (void*)M->loss, (void*)M->x,
+                    (void*)M->y, (void*)M->out, (void*)M->idx})
+        if (p) (void)hipFree(p);
+    if (M->stream) (void)hipStreamDestroy(M->stream);
+    delete M;
+}
+
+int g2_mlp_get_params(g2_mlp* M, float* out) {
+    if (!M || !out) return G2_EINVAL;
+    return hipMemcpy(out, M->params, kNParam * 4, hipMemcpyDeviceToHost) == hipSuccess ? G2_OK : G2_EHIP;
+}
+
+int g2_mlp_set_params(g2_mlp* M, const float* in) {
+    if (!M || !in) return G2_EINVAL;
+    return hipMemcpy(M->params, in, kNParam * 4, hipMemcpyHostToDevice) == hipSuccess ? G2_OK : G2_EHIP;
+}
+
+int g2_mlp_infer(g2_mlp* M, const float* x, uint64_t n, float* y) {
+    if (!M || (n && (!x || !y))) return G2_EINVAL;
+    if (n == 0) return G2_OK;
+    (void)hipSetDevice(M->device);
+    if (mlp_ensure(M, n)) return G2_EHIP;
+    if (hipMemcpyAsync(M->x, x, n * kIn * 4, hipMemcpyHostToDevice, M->stream)) return G2_EHIP;
+    MlpArgs a{M->params, M->x, nullptr, nullptr, n, 1.f, nullptr, nullptr, M->out};
+    const uint64_t blocks = std::min<uint64_t>((n + 63) / 64, (uint64_t)M->num_cus * 4);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
+    if (hipGetLastError() != hipSuccess) return G2_EHIP;
+    if (hipMemcpyAsync(y, M->out, n * 4, hipMemcpyDeviceToHost, M->stream)) return G2_EHIP;
+    return hipStreamSynchronize(M->stream) == hipSuccess ? G2_OK : G2_EHIP;
+}
+
+// gradient of mean squared error over the n samples at the current parameters
+int g2_mlp_grad(g2_mlp* M, const float* x, const float* y, uint64_t n, float* grad_out, float* loss_out) {
+    if (!M || !n || !x || !y || !grad_out) return G2_EINVAL;
+    (void)hipSetDevice(M->device);
+    if (mlp_ensure(M, n)) return G2_EHIP;
+    if (hipMemcpyAsync(M->x, x, n * kIn * 4, hipMemcpyHostToDevice, M->stream) ||
+        hipMemcpyAsync(M->y, y, n * 4, hipMemcpyHostToDevice, M->stream) ||
+        hipMemsetAsync(M->grad, 0, kNParam * 4, M->stream) || hipMemsetAsync(M->loss, 0, 4, M->stream))
+        return G2_EHIP;
+    MlpArgs a{M->params, M->x, M->y, nullptr, n, 1.f / n, M->grad, M->loss, nullptr};
+    const uint64_t blocks = std::min<uint64_t>((n + 63) / 64, (uint64_t)M->num_cus * 4);
+    hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
+    if (hipGetLastError() != hipSuccess) return G2_EHIP;
+    float sse = 0;
+    if (hipMemcpyAsync(grad_out, M->grad, kNParam * 4, hipMemcpyDeviceToHost, M->stream) ||
+        hipMemcpyAsync(&sse, M->loss, 4, hipMemcpyDeviceToHost, M->stream) || hipStreamSynchronize(M->stream))
+        return G2_EHIP;
+    if (loss_out) *loss_out = sse / n;
+    return G2_OK;
+}
+
+// Adam over shuffled mini-batches; *loss_out = mean squared error of the last epoch
+int g2_mlp_train(g2_mlp* M, const float* x, const float* y, uint64_t n, uint32_t epochs, uint32_t batch,
+                 float lr, float* loss_out) {
+    if (!M || !n || !x || !y || !batch) return G2_EINVAL;
+    (void)hipSetDevice(M->device);
+    if (mlp_ensure(M, n)) return G2_EHIP;
+    if (hipMemcpyAsync(M->x, x, n * kIn * 4, hipMemcpyHostToDevice, M->stream) ||
+        hipMemcpyAsync(M->y, y, n * 4, hipMemcpyHostToDevice, M->stream))
+        return G2_EHIP;
+    std::vector<uint32_t> perm(n);
+    for (uint64_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+    float last = 0;
+    for (uint32_t ep = 0; ep < epochs; ep++) {
+        std::shuffle(perm.begin(), perm.end(), M->rng);
+        if (hipMemcpyAsync(M->idx, perm.data(), n * 4, hipMemcpyHostToDevice, M->stream) ||
+            hipMemsetAsync(M->loss, 0, 4, M->stream))
+            return G2_EHIP;
+        for (uint64_t b0 = 0; b0 < n; b0 += batch) {
+            const uint64_t bn = std::min<uint64_t>(batch, n - b0);
+            if (hipMemsetAsync(M->grad, 0, kNParam * 4, M->stream)) return G2_EHIP;
+            MlpArgs a{M->params, M->x, M->y, M->idx + b0, bn, 1.f / bn, M->grad, M->loss, nullptr};
+            const uint64_t blocks = std::min<uint64_t>((bn + 63) / 64, (uint64_t)M->num_cus * 4);
+            hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
+            M->step++;
+            const float b1 = 0.9f, b2 = 0.999f;
+            AdamArgs ad{M->params, M->grad, M->m, M->v, lr, b1, b2, 1e-8f,
+                        1.f / (1.f - std::pow(b1, (float)M->step)), 1.f / (1.f - std::pow(b2, (float)M->step))};
+            hipLaunchKernelGGL(adam_kernel, dim3(8), dim3(256), 0, M->stream, ad);
+            if (hipGetLastError() != hipSuccess) return G2_EHIP;
+        }
+        float sse = 0;
+        if (hipMemcpyAsync(&sse, M->loss, 4, hipMemcpyDeviceToHost, M->stream) || hipStreamSynchronize(M->stream))
+            return G2_EHIP;
+        last = sse / n;
+    }
+    if (loss_out) *loss_out = last;
+    return G2_OK;
+}
+
+}  // extern "C"

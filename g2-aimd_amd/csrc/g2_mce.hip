@@ -51,6 +51,7 @@ struct MceArgs {
     unsigned long long* hist;
     uint32_t hist_len;
     unsigned long long* per_root;        // nullable
+    unsigned long long* wave_span;       // per wave: first task start, last task end (GPU real-time clock)
 };
 
 // u64 words of the per-root arena: adjP (d*W) + adjX (p0*WX) + (p0+1) frames
@@ -79,6 +80,8 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     uint64_t* ws = a.scratch + gwave * a.scratch_words;
     uint64_t my_cliques = 0, my_nodes = 0, my_bytes = 0;
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_end = t_start;
 
     while (true) {
         unsigned long long t = 0;
@@ -313,6 +316,11 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         }
         my_nodes += root_nodes;
         if (a.per_root && lane == 0 && root_nodes) atomicAdd(&a.per_root[si], (unsigned long long)root_nodes);
+        t_end = wall_clock64();
+    }
+    if (a.wave_span && lane == 0) {
+        a.wave_span[2 * gwave] = t_start;
+        a.wave_span[2 * gwave + 1] = t_end;
     }
 
     // ---- flush per-wave and per-block tallies ----------------------------
@@ -339,6 +347,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     if ((rc = ensure(c, c->counters, CTR_N * 8, "counters"))) return rc;
     if ((rc = ensure(c, c->hist, (uint64_t)hl * 8, "histogram"))) return rc;
     if ((rc = ensure(c, c->scratch, nwaves * sw * 8, "wave arenas"))) return rc;
+    if ((rc = ensure(c, c->spans, nwaves * 16, "wave spans"))) return rc;
     if (per_root_nodes && (rc = ensure(c, c->per_root, std::max<uint64_t>(c->ns, 1) * 8, "per-root counts"))) return rc;
     // task list: sources in schedule order, heavy ones dealt into parts
     const uint64_t* tasks = nullptr;
@@ -382,6 +391,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     a.hist = (unsigned long long*)c->hist.ptr;
     a.hist_len = hl;
     a.per_root = per_root_nodes ? (unsigned long long*)c->per_root.ptr : nullptr;
+    a.wave_span = (unsigned long long*)c->spans.ptr;
 
     auto launch = [&](uint64_t blocks) {
         if (account) hipLaunchKernelGGL(mce_kernel<true>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);
@@ -395,9 +405,22 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     if ((e = launch(nblocks))) return hip_err(c, e, "mce launch");
     if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
     unsigned long long h_ctr[CTR_N];
+    std::vector<unsigned long long> spans(2 * nwaves);
     if ((e = hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
+    if ((e = hipMemcpyAsync(spans.data(), c->spans.ptr, nwaves * 16, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H spans");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
     (void)hipEventElapsedTime(&k1, c->ev[1], c->ev[2]);
+    double idle = 0;
+    {
+        unsigned long long lo = ~0ull, hi = 0;
+        double busy = 0;
+        for (uint64_t w = 0; w < nwaves; w++) {
+            lo = std::min(lo, spans[2 * w]);
+            hi = std::max(hi, spans[2 * w + 1]);
+            busy += (double)(spans[2 * w + 1] - spans[2 * w]);
+        }
+        if (hi > lo) idle = 1.0 - busy / ((double)nwaves * (double)(hi - lo));
+    }
     const uint64_t nbig = h_ctr[CTR_NBIG];
     if (nbig) {
         // large-root pass: arenas sized for the largest deferred root
@@ -412,6 +435,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         a.scratch = (uint64_t*)c->big_scratch.ptr;
         a.scratch_words = need;
         a.big_list = nullptr;
+        a.wave_span = nullptr;
         if ((e = hipMemsetAsync(ctr + CTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
         if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
         if ((e = launch(bwaves / bw))) return hip_err(c, e, "mce large-root launch");
@@ -438,6 +462,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         st->alg_bytes = account ? h_ctr[CTR_BYTES] : 0;
         st->big_roots = nbig;
         st->max_scratch_words = h_ctr[CTR_MAXNEED];
+        st->wave_idle_frac = idle;
     }
     return G2_OK;
 }

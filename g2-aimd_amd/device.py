@@ -26,7 +26,11 @@ G2_RUN_CANONICAL = 2
 # every symbol include/g2hip.h declares
 EXPORTS = ("g2_create", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
            "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
-           "g2_pinned_free", "g2_plan_local_eligible")
+           "g2_pinned_free", "g2_plan_local_eligible", "g2_features", "g2_mlp_create",
+           "g2_mlp_destroy", "g2_mlp_get_params", "g2_mlp_set_params", "g2_mlp_infer", "g2_mlp_grad",
+           "g2_mlp_train")
+G2_NFEAT_INT = 5
+G2_MLP_NPARAM = 4801
 
 
 class Tuning(C.Structure):
@@ -38,7 +42,7 @@ class Tuning(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("launches", C.c_uint32), ("total_ms", C.c_double),
                 ("nodes", C.c_uint64), ("alg_bytes", C.c_uint64), ("big_roots", C.c_uint64),
-                ("max_scratch_words", C.c_uint64)]
+                ("max_scratch_words", C.c_uint64), ("wave_idle_frac", C.c_double)]
 
     def to_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +78,17 @@ def hip_lib() -> C.CDLL:
     L.g2_device_info.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.c_char_p, C.c_uint32]
     L.g2_plan_local_eligible.argtypes = [C.POINTER(G2Plan)]
+    f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.g2_features.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    L.g2_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
+    L.g2_mlp_destroy.argtypes = [vp]
+    L.g2_mlp_destroy.restype = None
+    L.g2_mlp_get_params.argtypes = [vp, f32p]
+    L.g2_mlp_set_params.argtypes = [vp, f32p]
+    L.g2_mlp_infer.argtypes = [vp, f32p, C.c_uint64, f32p]
+    L.g2_mlp_grad.argtypes = [vp, f32p, f32p, C.c_uint64, f32p, C.POINTER(C.c_float)]
+    L.g2_mlp_train.argtypes = [vp, f32p, f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
+                               C.POINTER(C.c_float)]
     _hip = L
     return L
 
@@ -189,3 +204,70 @@ class Executor:
         if per_root:
             out["per_root_count"] = prc[: self.ns]
         return out
+
+    def features(self) -> np.ndarray:
+        """Integer root features (ns x 5): degree, 2-paths, triangles,
+        neighbours sharing a neighbour with the root, later neighbours."""
+        out = np.zeros((max(self.ns, 1), G2_NFEAT_INT), np.int64)
+        self._check(hip_lib().g2_features(self._ctx, out), "g2_features")
+        return out[: self.ns]
+
+
+class Mlp:
+    """Cost model 8 -> 64 -> 64 -> 1 trained and run by the matrix-core kernels."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self._h = C.c_void_p()
+        if hip_lib().g2_mlp_create(C.byref(self._h), device, seed) != 0:
+            raise RuntimeError("g2_mlp_create failed")
+
+    def close(self):
+        if self._h:
+            hip_lib().g2_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _x(x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        assert x.ndim == 2 and x.shape[1] == 8
+        return x
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(G2_MLP_NPARAM, np.float32)
+        assert hip_lib().g2_mlp_get_params(self._h, p) == 0
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert len(p) == G2_MLP_NPARAM and hip_lib().g2_mlp_set_params(self._h, p) == 0
+
+    def infer(self, x) -> np.ndarray:
+        x = self._x(x)
+        y = np.zeros(max(len(x), 1), np.float32)
+        if len(x):
+            assert hip_lib().g2_mlp_infer(self._h, x, len(x), y) == 0
+        return y[: len(x)]
+
+    def grad(self, x, y):
+        x = self._x(x)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(G2_MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        assert hip_lib().g2_mlp_grad(self._h, x, y, len(x), g, C.byref(loss)) == 0
+        return loss.value, g
+
+    def train(self, x, y, epochs=20, batch=1024, lr=1e-3) -> float:
+        x = self._x(x)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = C.c_float()
+        rc = hip_lib().g2_mlp_train(self._h, x, y, len(x), epochs, batch, lr, C.byref(loss))
+        if rc != 0:
+            raise RuntimeError(f"g2_mlp_train failed ({rc})")
+        return loss.value
+

@@ -60,6 +60,8 @@ typedef struct g2_stats {
     uint64_t alg_bytes;           /* modelled algorithmic bytes (only with G2_RUN_ACCOUNT) */
     uint64_t big_roots;           /* roots routed to the large-root pass */
     uint64_t max_scratch_words;   /* largest per-root scratch need seen */
+    double   wave_idle_frac;      /* MCE: 1 - sum of per-wave busy spans / (waves x makespan),
+                                     from the GPU real-time clock (tail idleness) */
 } g2_stats;
 
 /* root schedule for g2_mce_run (NULL = the given source order, one wave per root) */
@@ -110,6 +112,31 @@ int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_
  * every automorphism class for every ranking (so the total does not depend on
  * the vertex order).  0 otherwise; < 0 for an invalid plan.  Host only. */
 int  g2_plan_local_eligible(const g2_plan* plan);
+
+/* ---- learned root scheduler (no counterpart in the reference) ----------- */
+
+/* Integer features of every uploaded source, G2_NFEAT_INT per source:
+ * degree, 2-path count sum_{u in N(v)} (deg(u)-1), triangles at v, neighbours
+ * sharing a common neighbour with v, and |{u in N(v): u > v}| (the root's P). */
+#define G2_NFEAT_INT 5
+int  g2_features(g2_ctx* ctx, int64_t* out);
+
+/* Cost model: MLP 8 -> 64 -> 64 -> 1 (ReLU), BF16 weights/activations on the
+ * matrix cores, FP32 accumulation, FP32 master weights, Adam.  Parameters
+ * (fp32): W1[8][64], b1[64], W2[64][64], b2[64], W3[64], b3. */
+#define G2_MLP_NPARAM 4801
+typedef struct g2_mlp g2_mlp;
+int  g2_mlp_create(g2_mlp** mlp, int device, uint64_t seed);
+void g2_mlp_destroy(g2_mlp* mlp);
+int  g2_mlp_get_params(g2_mlp* mlp, float* out);
+int  g2_mlp_set_params(g2_mlp* mlp, const float* in);
+/* y[i] = f(x[i*8 .. i*8+7]) */
+int  g2_mlp_infer(g2_mlp* mlp, const float* x, uint64_t n, float* y);
+/* gradient of the mean squared error over n samples (no update) */
+int  g2_mlp_grad(g2_mlp* mlp, const float* x, const float* y, uint64_t n, float* grad, float* loss);
+/* Adam over seeded shuffled mini-batches; *loss = MSE of the last epoch */
+int  g2_mlp_train(g2_mlp* mlp, const float* x, const float* y, uint64_t n, uint32_t epochs,
+                  uint32_t batch, float lr, float* loss);
 
 /* Device properties used for launch sizing */
 int  g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* name, uint32_t name_len);
