@@ -113,7 +113,8 @@ void g2_destroy(g2_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
-                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans})
+                      &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans,
+                      &c->batch_off, &c->batch_progs, &c->batch_q})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -158,8 +159,11 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     // [off[i], off[i+1]) of gm_chunk row entries each (one window when 0)
     c->gm_task_chunk = c->tune.gm_chunk;
     c->h_gm_task_off.assign(ns + 1, 0);
+    c->h_src_deg.resize(ns);
+    c->batch_off_valid = false;
     for (uint64_t i = 0; i < ns; i++) {
         const uint64_t d = rp[src[i] + 1] - rp[src[i]];
+        c->h_src_deg[i] = d;
         uint64_t w = 1;
         if (c->gm_task_chunk && d > c->gm_task_chunk) w = (d + c->gm_task_chunk - 1) / c->gm_task_chunk;
         c->h_gm_task_off[i + 1] = c->h_gm_task_off[i] + w;
@@ -192,6 +196,16 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     return mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);
+}
+
+int g2_gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
+                    uint64_t* nodes, uint32_t* over_budget, g2_stats* st) {
+    if (!c) return G2_EINVAL;
+    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (nq && !plans) return set_err(c, G2_EINVAL, "null plans");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    return gm_run_batch(c, plans, nq, cfg, counts, nodes, over_budget, st);
 }
 
 int g2_plan_local_eligible(const g2_plan* p) {

@@ -73,6 +73,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Next task of a persistent loop: lane 0 takes it from the counter and the
+// whole wave gets it.  The wave barrier (a convergent operation) pins the
+// fetch at a point where the wave is converged.  Without it the compiler may
+// thread a "lane 0 only" branch at the end of one iteration (a result store)
+// into the next iteration's "lane 0 only" fetch; the structurised loop then
+// lets lanes 1..63 go round without passing the fetch, reading a stale index
+// forever.
+__device__ __forceinline__ uint64_t fetch_task(unsigned long long* ctr) {
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long t = 0;
+    if (lane_id() == 0) t = atomicAdd(ctr, 1ull);
+    return uniform((uint64_t)t);
+}
+
 // first index in [lo,hi) whose value is >= val (hi if none)
 __device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ cols, uint64_t lo, uint64_t hi, uint32_t val) {
     while (lo < hi) {

@@ -37,7 +37,6 @@ struct GmArgs {
     uint64_t ns;
     uint64_t ntasks;
     uint32_t chunk;                         // level-1 window length (0 = whole row)
-    GmProgram prog;
     uint32_t* scratch;
     uint64_t cap;                           // entries per list
     unsigned long long* ctr;
@@ -187,140 +186,182 @@ __device__ inline uint32_t min_row(const GmArgs& a, const WaveState& S, uint32_t
     return best;
 }
 
+// Per-query node budget of the batched launch: a wave charges its nodes to
+// the query's counter in blocks of kBudgetBlock; once the total passes the
+// limit the query's flag is raised and every wave working on it drops it.
+struct NodeBudget {
+    unsigned long long* qnodes;
+    unsigned int* qflag;
+    uint64_t limit;
+};
+
+constexpr uint64_t kBudgetBlock = 256;
+
+// charge `pending` nodes of query q; true when the query is over budget
+__device__ inline bool budget_charge(const NodeBudget& nb, uint32_t q, uint64_t& pending) {
+    __builtin_amdgcn_wave_barrier();  // converged here (see fetch_task)
+    uint32_t over = 0;
+    if (lane_id() == 0) {
+        const unsigned long long old = atomicAdd(&nb.qnodes[q], (unsigned long long)pending);
+        if (old + pending > nb.limit) atomicOr(&nb.qflag[q], 1u);
+        over = __hip_atomic_load(&nb.qflag[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    pending = 0;
+    return uniform(over) != 0;
+}
+
+// One root task: source r, level-1 window `win`, under program P.  The
+// wave's lists live in `ws` (L(id) at id*cap, PRE(id) at (levels+id)*cap).
+// Returns false when the query's node budget ran out (BUDGET only).
+template <bool ACCOUNT, bool BUDGET>
+__device__ bool gm_root(const GmArgs& a, const GmProgram& P, WaveState& S, uint32_t* ws, uint32_t levels,
+                        uint64_t r, uint64_t win, uint64_t& lane_total, uint64_t& bytes, uint64_t& nodes,
+                        const NodeBudget& nb, uint32_t q, uint64_t& pending) {
+    const uint32_t lane = lane_id();
+    const uint32_t n = P.n;
+    const uint64_t cap = a.cap;
+    auto L = [&](uint32_t id) { return ws + (uint64_t)id * cap; };
+    auto PRE = [&](uint32_t id) { return ws + (uint64_t)(levels + id) * cap; };
+    const uint32_t v0 = uniform(a.src[r]);
+    const uint64_t rlo = a.rp[v0], rhi = a.rp[v0 + 1];
+    uint64_t wlo = rlo, whi = rhi;
+    if (a.chunk) {
+        wlo = min(rhi, rlo + win * a.chunk);
+        whi = min(rhi, wlo + a.chunk);
+    }
+    if (lane == 0) S.M[0] = v0;
+    wave_sync();
+    if (ACCOUNT && win == 0 && lane == 0) bytes += 4 + 4 * 1 + 4 * (rhi - rlo);
+
+    // position 1: the root's row (window)
+    {
+        const GmStep& st = P.step[1];
+        const Range rg = cond_range(S, st.ncond, st.cop, st.cpos);
+        uint64_t nout = 0, bsum = 0, npass = 0;
+        nodes++;
+        if (BUDGET) pending++;
+        if (n == 2) {
+            filter_row<ACCOUNT>(a, S, st.nbn, st.bn, 0, wlo, whi, rg, st.ncond, st.cop, st.cpos, nullptr,
+                                &lane_total, &bsum, &npass);
+        } else {
+            nout = filter_row<ACCOUNT>(a, S, st.nbn, st.bn, 0, wlo, whi, rg, st.ncond, st.cop, st.cpos, L(1),
+                                       nullptr, &bsum, &npass);
+            if (ACCOUNT && lane == 0) {
+                bsum += store_bytes(st, 1, n, nout);
+                // a PREFIX store writes the prefix once per root, not per window
+                if (win && st.store_prefix) bsum -= 4;
+            }
+            if (lane == 0) { S.len[1] = nout; S.idx[1] = 0; }
+        }
+        if (ACCOUNT && lane == 0) bytes += bsum;
+    }
+    wave_sync();
+    if (n <= 2) return true;
+    // pre-intersection of position 2 if shared (depends on position 0 only)
+    uint32_t depth = 1;
+    auto build_pre = [&](uint32_t id, bool charge) {
+        // prefix = positions < id-1 are matched
+        const GmStep& st = P.step[id];
+        if (!ACCOUNT && st.pre_is_prev) {  // PRE(id) is L(id-1), already built
+            wave_sync();
+            return;
+        }
+        const uint32_t parent = min_row(a, S, st.npre, st.pre);
+        const uint32_t pm = S.M[parent];
+        const Range rg = cond_range(S, st.npcond, st.pcop, st.pcpos);
+        uint64_t bsum = 0;
+        const uint64_t len = filter_row<ACCOUNT>(a, S, st.npre, st.pre, parent, a.rp[pm], a.rp[pm + 1], rg,
+                                                 st.npcond, st.pcop, st.pcpos, PRE(id), nullptr, &bsum, nullptr);
+        if (lane == 0) {
+            S.prelen[id] = len;
+            if (ACCOUNT && charge) bytes += bsum + 4 * (id - 1) + 4 * (a.rp[pm + 1] - a.rp[pm]);
+        }
+        wave_sync();
+    };
+    if (P.step[2].shared) build_pre(2, win == 0);  // same list for every window
+    while (true) {
+        const uint32_t id = depth;
+        const uint64_t ix = S.idx[id];
+        if (ix >= S.len[id]) {
+            if (depth == 1) break;
+            depth--;
+            continue;
+        }
+        const uint32_t cand = L(id)[ix];
+        if (lane == 0) { S.M[id] = cand; S.idx[id] = ix + 1; }
+        wave_sync();
+        const uint32_t nx = id + 1;
+        const GmStep& st = P.step[nx];
+        nodes++;
+        if (BUDGET && ++pending >= kBudgetBlock && budget_charge(nb, q, pending)) return false;
+        uint64_t bsum = 0;
+        if (st.shared) {
+            const bool alias = !ACCOUNT && st.pre_is_prev;
+            const uint32_t* pl = alias ? L(id) : PRE(nx);
+            const uint64_t plen = alias ? S.len[id] : S.prelen[nx];
+            if (nx + 1 == n) {
+                filter_pre<ACCOUNT>(a, S, st, pl, plen, nullptr, &lane_total, &bsum);
+            } else {
+                const uint64_t nout = filter_pre<ACCOUNT>(a, S, st, pl, plen, L(nx), nullptr, &bsum);
+                if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
+                if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
+            }
+        } else {
+            const uint32_t parent = min_row(a, S, st.nbn, st.bn);
+            const uint32_t pm = S.M[parent];
+            const Range rg = cond_range(S, st.ncond, st.cop, st.cpos);
+            if (ACCOUNT && lane == 0) bsum += 4 * nx + 4 * (a.rp[pm + 1] - a.rp[pm]);
+            if (nx + 1 == n) {
+                filter_row<ACCOUNT>(a, S, st.nbn, st.bn, parent, a.rp[pm], a.rp[pm + 1], rg, st.ncond, st.cop,
+                                    st.cpos, nullptr, &lane_total, &bsum, nullptr);
+            } else {
+                const uint64_t nout = filter_row<ACCOUNT>(a, S, st.nbn, st.bn, parent, a.rp[pm], a.rp[pm + 1],
+                                                          rg, st.ncond, st.cop, st.cpos, L(nx), nullptr, &bsum,
+                                                          nullptr);
+                if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
+                if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
+            }
+        }
+        if (ACCOUNT && lane == 0) bytes += bsum;
+        wave_sync();
+        if (nx + 1 < n) {
+            if (P.step[nx + 1].shared) build_pre(nx + 1, true);
+            depth = nx;
+        }
+    }
+    return true;
+}
+
+// task t of a query -> (source, level-1 window)
+__device__ inline uint64_t task_source(const GmArgs& a, uint64_t t, uint64_t* win) {
+    uint64_t lo = 0, hi = a.ns;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a.task_off[mid] <= t) lo = mid; else hi = mid;
+    }
+    const uint64_t r = uniform(lo);
+    *win = t - a.task_off[r];
+    return r;
+}
+
 template <bool ACCOUNT>
-__global__ __launch_bounds__(256) void gm_kernel(GmArgs a) {
+__global__ __launch_bounds__(256) void gm_kernel(GmArgs a, GmProgram prog) {
     __shared__ WaveState states[kMaxBlockWaves];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x / kWave;
     WaveState& S = states[wib];
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wib;
-    const uint32_t n = a.prog.n;
-    const uint64_t cap = a.cap;
-    uint32_t* ws = a.scratch + gwave * cap * (2 * G2_MAXQ);
-    auto L = [&](uint32_t id) { return ws + (uint64_t)id * cap; };
-    auto PRE = [&](uint32_t id) { return ws + (uint64_t)(G2_MAXQ + id) * cap; };
-    uint64_t lane_total = 0, bytes = 0, nodes = 0;
+    uint32_t* ws = a.scratch + gwave * a.cap * (2 * prog.n);
+    uint64_t lane_total = 0, bytes = 0, nodes = 0, pending = 0;
+    const NodeBudget nb{nullptr, nullptr, 0};
 
     while (true) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(&a.ctr[GCTR_NEXT], 1ull);
-        t = uniform((uint64_t)t);
+        const uint64_t t = fetch_task(&a.ctr[GCTR_NEXT]);
         if (t >= a.ntasks) break;
-        // task -> (source, level-1 window)
-        uint64_t lo = 0, hi = a.ns;
-        while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) >> 1;
-            if (a.task_off[mid] <= t) lo = mid; else hi = mid;
-        }
-        const uint64_t r = uniform(lo);
-        const uint64_t win = t - a.task_off[r];
-        const uint32_t v0 = uniform(a.src[r]);
-        const uint64_t rlo = a.rp[v0], rhi = a.rp[v0 + 1];
-        uint64_t wlo = rlo, whi = rhi;
-        if (a.chunk) {
-            wlo = min(rhi, rlo + win * a.chunk);
-            whi = min(rhi, wlo + a.chunk);
-        }
-        if (lane == 0) S.M[0] = v0;
-        wave_sync();
+        uint64_t win;
+        const uint64_t r = task_source(a, t, &win);
         const uint64_t before = lane_total;
-        if (ACCOUNT && win == 0 && lane == 0) bytes += 4 + 4 * 1 + 4 * (rhi - rlo);
-
-        // position 1: the root's row (window)
-        {
-            const GmStep& st = a.prog.step[1];
-            const Range rg = cond_range(S, st.ncond, st.cop, st.cpos);
-            uint64_t nout = 0, bsum = 0, npass = 0;
-            nodes++;
-            if (n == 2) {
-                filter_row<ACCOUNT>(a, S, st.nbn, st.bn, 0, wlo, whi, rg, st.ncond, st.cop, st.cpos, nullptr,
-                                    &lane_total, &bsum, &npass);
-            } else {
-                nout = filter_row<ACCOUNT>(a, S, st.nbn, st.bn, 0, wlo, whi, rg, st.ncond, st.cop, st.cpos, L(1),
-                                           nullptr, &bsum, &npass);
-                if (ACCOUNT && lane == 0) {
-                    bsum += store_bytes(st, 1, n, nout);
-                    // a PREFIX store writes the prefix once per root, not per window
-                    if (win && st.store_prefix) bsum -= 4;
-                }
-                if (lane == 0) { S.len[1] = nout; S.idx[1] = 0; }
-            }
-            if (ACCOUNT && lane == 0) bytes += bsum;
-        }
-        wave_sync();
-        if (n > 2) {
-            // pre-intersection of position 2 if shared (depends on position 0 only)
-            uint32_t depth = 1;
-            auto build_pre = [&](uint32_t id, bool charge) {
-                // prefix = positions < id-1 are matched
-                const GmStep& st = a.prog.step[id];
-                if (!ACCOUNT && st.pre_is_prev) {  // PRE(id) is L(id-1), already built
-                    wave_sync();
-                    return;
-                }
-                const uint32_t parent = min_row(a, S, st.npre, st.pre);
-                const uint32_t pm = S.M[parent];
-                const Range rg = cond_range(S, st.npcond, st.pcop, st.pcpos);
-                uint64_t bsum = 0;
-                const uint64_t len = filter_row<ACCOUNT>(a, S, st.npre, st.pre, parent, a.rp[pm], a.rp[pm + 1], rg,
-                                                         st.npcond, st.pcop, st.pcpos, PRE(id), nullptr, &bsum, nullptr);
-                if (lane == 0) {
-                    S.prelen[id] = len;
-                    if (ACCOUNT && charge) bytes += bsum + 4 * (id - 1) + 4 * (a.rp[pm + 1] - a.rp[pm]);
-                }
-                wave_sync();
-            };
-            if (a.prog.step[2].shared) build_pre(2, win == 0);  // same list for every window
-            while (true) {
-                const uint32_t id = depth;
-                const uint64_t ix = S.idx[id];
-                if (ix >= S.len[id]) {
-                    if (depth == 1) break;
-                    depth--;
-                    continue;
-                }
-                const uint32_t cand = L(id)[ix];
-                if (lane == 0) { S.M[id] = cand; S.idx[id] = ix + 1; }
-                wave_sync();
-                const uint32_t nx = id + 1;
-                const GmStep& st = a.prog.step[nx];
-                nodes++;
-                uint64_t bsum = 0;
-                if (st.shared) {
-                    const bool alias = !ACCOUNT && st.pre_is_prev;
-                    const uint32_t* pl = alias ? L(id) : PRE(nx);
-                    const uint64_t plen = alias ? S.len[id] : S.prelen[nx];
-                    if (nx + 1 == n) {
-                        filter_pre<ACCOUNT>(a, S, st, pl, plen, nullptr, &lane_total, &bsum);
-                    } else {
-                        const uint64_t nout = filter_pre<ACCOUNT>(a, S, st, pl, plen, L(nx), nullptr, &bsum);
-                        if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
-                        if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
-                    }
-                } else {
-                    const uint32_t parent = min_row(a, S, st.nbn, st.bn);
-                    const uint32_t pm = S.M[parent];
-                    const Range rg = cond_range(S, st.ncond, st.cop, st.cpos);
-                    if (ACCOUNT && lane == 0) bsum += 4 * nx + 4 * (a.rp[pm + 1] - a.rp[pm]);
-                    if (nx + 1 == n) {
-                        filter_row<ACCOUNT>(a, S, st.nbn, st.bn, parent, a.rp[pm], a.rp[pm + 1], rg, st.ncond, st.cop,
-                                            st.cpos, nullptr, &lane_total, &bsum, nullptr);
-                    } else {
-                        const uint64_t nout = filter_row<ACCOUNT>(a, S, st.nbn, st.bn, parent, a.rp[pm], a.rp[pm + 1],
-                                                                  rg, st.ncond, st.cop, st.cpos, L(nx), nullptr, &bsum,
-                                                                  nullptr);
-                        if (ACCOUNT && lane == 0) bsum += store_bytes(st, nx, n, nout);
-                        if (lane == 0) { S.len[nx] = nout; S.idx[nx] = 0; }
-                    }
-                }
-                if (ACCOUNT && lane == 0) bytes += bsum;
-                wave_sync();
-                if (nx + 1 < n) {
-                    if (a.prog.step[nx + 1].shared) build_pre(nx + 1, true);
-                    depth = nx;
-                }
-            }
-        }
+        gm_root<ACCOUNT, false>(a, prog, S, ws, prog.n, r, win, lane_total, bytes, nodes, nb, 0, pending);
         if (a.per_root) {
             const uint64_t mine = wave_sum(lane_total - before);
             if (lane == 0 && mine) atomicAdd(&a.per_root[r], (unsigned long long)mine);
@@ -333,6 +374,57 @@ __global__ __launch_bounds__(256) void gm_kernel(GmArgs a) {
         if (ACCOUNT && bytes) atomicAdd(&a.ctr[GCTR_BYTES], (unsigned long long)bytes);
         atomicAdd(&a.ctr[GCTR_NODES], (unsigned long long)nodes);
     }
+}
+
+// Many plans in one persistent launch.  Task t is query order[t / per_q]'s
+// block t % per_q of `rpt` consecutive root tasks; its matches go to that
+// query's counter, so each count is the sum over the same root tasks the
+// solo launch runs.
+struct GmBatchArgs {
+    const GmProgram* __restrict__ progs;
+    const uint32_t* __restrict__ order;
+    uint32_t nq;
+    uint32_t levels;             // list levels per wave arena (largest plan size)
+    uint32_t rpt;                // root tasks per batch task
+    uint64_t per_q;              // batch tasks per query
+    unsigned long long* qcount;
+    NodeBudget nb;               // qnodes always counted; limit ~0 = none
+};
+
+__global__ __launch_bounds__(256) void gm_batch_kernel(GmArgs a, GmBatchArgs b) {
+    __shared__ WaveState states[kMaxBlockWaves];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x / kWave;
+    WaveState& S = states[wib];
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wib;
+    uint32_t* ws = a.scratch + gwave * a.cap * (2 * b.levels);
+    const uint64_t total = (uint64_t)b.nq * b.per_q;
+    uint64_t nodes_all = 0, bytes = 0;
+
+    while (true) {
+        const uint64_t t = fetch_task(&a.ctr[GCTR_NEXT]);
+        if (t >= total) break;
+        const uint32_t q = uniform(b.order[t / b.per_q]);
+        uint32_t over = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) over = __hip_atomic_load(&b.nb.qflag[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (uniform(over)) continue;
+        const GmProgram& P = b.progs[q];
+        const uint64_t t0 = (t % b.per_q) * b.rpt;
+        const uint64_t t1 = min(a.ntasks, t0 + b.rpt);
+        uint64_t lane_total = 0, nodes = 0, pending = 0;
+        bool ok = true;
+        for (uint64_t rt = t0; rt < t1 && ok; rt++) {
+            uint64_t win;
+            const uint64_t r = task_source(a, rt, &win);
+            ok = gm_root<false, true>(a, P, S, ws, b.levels, r, win, lane_total, bytes, nodes, b.nb, q, pending);
+        }
+        if (ok && pending) budget_charge(b.nb, q, pending);
+        nodes_all += nodes;
+        const uint64_t tot = wave_sum(lane_total);
+        if (lane == 0 && tot) atomicAdd(&b.qcount[q], (unsigned long long)tot);
+    }
+    if (lane == 0) atomicAdd(&a.ctr[GCTR_NODES], (unsigned long long)nodes_all);
 }
 
 int gm_compile(g2_ctx* c, const g2_plan* p) {
@@ -421,7 +513,7 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_st
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
-    const uint64_t per_wave = cap * 2 * G2_MAXQ * 4;
+    const uint64_t per_wave = cap * 2 * c->prog.n * 4;
     // shrink the grid until the arenas fit the budget
     while (nblocks > 1 && c->budget && c->used - c->scratch.bytes + nblocks * bw * per_wave > c->budget) nblocks /= 2;
     int rc;
@@ -440,14 +532,13 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_st
     a.ns = c->ns;
     a.ntasks = c->h_gm_task_off.empty() ? 0 : c->h_gm_task_off.back();
     a.chunk = c->gm_task_chunk;
-    a.prog = c->prog;
     a.scratch = (uint32_t*)c->scratch.ptr;
     a.cap = cap;
     a.ctr = ctr;
     a.per_root = per_root ? (unsigned long long*)c->per_root.ptr : nullptr;
     if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
-    if (account) hipLaunchKernelGGL(gm_kernel<true>, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a);
-    else hipLaunchKernelGGL(gm_kernel<false>, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a);
+    if (account) hipLaunchKernelGGL(gm_kernel<true>, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a, c->prog);
+    else hipLaunchKernelGGL(gm_kernel<false>, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a, c->prog);
     if ((e = hipGetLastError())) return hip_err(c, e, "gm launch");
     if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
     unsigned long long h[GCTR_N];
@@ -465,6 +556,146 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_st
         st->nodes = h[GCTR_NODES];
         st->alg_bytes = account ? h[GCTR_BYTES] : 0;
         st->big_roots = 0;
+        st->max_scratch_words = cap;
+    }
+    return G2_OK;
+}
+
+}  // namespace g2
+
+namespace g2 {
+
+// level-1 window offsets of the sources for window length w (0 = one task per source)
+static int batch_task_offsets(g2_ctx* c, uint32_t w, const uint64_t** dev_off, uint64_t* ntasks) {
+    if (w == c->gm_task_chunk) {
+        *dev_off = (const uint64_t*)c->gm_task_off.ptr;
+        *ntasks = c->h_gm_task_off.back();
+        return G2_OK;
+    }
+    if (!c->batch_off_valid || c->batch_window != w) {
+        std::vector<uint64_t> off(c->ns + 1, 0);
+        for (uint64_t i = 0; i < c->ns; i++) {
+            const uint64_t d = c->h_src_deg[i];
+            off[i + 1] = off[i] + ((w && d > w) ? (d + w - 1) / w : 1);
+        }
+        int rc;
+        if ((rc = ensure(c, c->batch_off, (c->ns + 1) * 8, "batch task offsets"))) return rc;
+        hipError_t e = hipMemcpy(c->batch_off.ptr, off.data(), (c->ns + 1) * 8, hipMemcpyHostToDevice);
+        if (e) return hip_err(c, e, "H2D batch task offsets");
+        c->batch_window = w;
+        c->batch_ntasks = off.back();
+        c->batch_off_valid = true;
+    }
+    *dev_off = (const uint64_t*)c->batch_off.ptr;
+    *ntasks = c->batch_ntasks;
+    return G2_OK;
+}
+
+int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
+                 uint64_t* qnodes, uint32_t* over, g2_stats* st) {
+    const g2_batch dflt{};
+    if (!cfg) cfg = &dflt;
+    if (st) *st = g2_stats{};
+    if (nq == 0) return G2_OK;
+    std::vector<GmProgram> progs(nq);
+    uint32_t levels = 2;
+    for (uint32_t q = 0; q < nq; q++) {
+        g2_ctx tmp;  // plan compilation only; no device state
+        if (gm_compile(&tmp, &plans[q]) != G2_OK)
+            return set_err(c, G2_EINVAL, "plan " + std::to_string(q) + ": " + tmp.err);
+        progs[q] = tmp.prog;
+        levels = std::max(levels, tmp.prog.n);
+    }
+    std::vector<uint32_t> order(nq);
+    if (cfg->order) {
+        std::vector<uint8_t> seen(nq, 0);
+        for (uint32_t i = 0; i < nq; i++) {
+            const uint32_t o = cfg->order[i];
+            if (o >= nq || seen[o]) return set_err(c, G2_EINVAL, "query order is not a permutation");
+            seen[o] = 1;
+            order[i] = o;
+        }
+    } else {
+        for (uint32_t i = 0; i < nq; i++) order[i] = i;
+    }
+    const uint64_t* task_off = nullptr;
+    uint64_t ntasks = 0;
+    int rc;
+    if ((rc = batch_task_offsets(c, cfg->window, &task_off, &ntasks))) return rc;
+    const uint32_t rpt = std::max<uint32_t>(cfg->roots_per_task, 1);
+    const uint64_t per_q = (ntasks + rpt - 1) / rpt;
+    const uint32_t wpc = cfg->waves_per_cu ? cfg->waves_per_cu : c->tune.waves_per_cu;
+    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
+    uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * wpc / bw);
+    const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
+    const uint64_t per_wave = cap * 2 * levels * 4;
+    while (nblocks > 1 && c->budget && c->used - c->scratch.bytes + nblocks * bw * per_wave > c->budget) nblocks /= 2;
+    if ((rc = ensure(c, c->counters, GCTR_N * 8, "counters"))) return rc;
+    if ((rc = ensure(c, c->scratch, nblocks * bw * per_wave, "wave arenas"))) return rc;
+    if ((rc = ensure(c, c->batch_progs, (uint64_t)nq * sizeof(GmProgram), "batch programs"))) return rc;
+    // per query: order u32, count u64, nodes u64, flag u32
+    const uint64_t qbytes = (uint64_t)nq * 24;
+    if ((rc = ensure(c, c->batch_q, qbytes, "batch query state"))) return rc;
+    auto* ctr = (unsigned long long*)c->counters.ptr;
+    auto* qcount = (unsigned long long*)c->batch_q.ptr;
+    auto* qn = qcount + nq;
+    auto* qord = (uint32_t*)(qn + nq);
+    auto* qflag = qord + nq;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(c->batch_progs.ptr, progs.data(), (uint64_t)nq * sizeof(GmProgram), hipMemcpyHostToDevice,
+                            c->stream)))
+        return hip_err(c, e, "H2D batch programs");
+    if ((e = hipMemcpyAsync(qord, order.data(), (uint64_t)nq * 4, hipMemcpyHostToDevice, c->stream)))
+        return hip_err(c, e, "H2D query order");
+    if ((e = hipMemsetAsync(ctr, 0, GCTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
+    if ((e = hipMemsetAsync(qcount, 0, (uint64_t)nq * 16, c->stream))) return hip_err(c, e, "memset query counts");
+    if ((e = hipMemsetAsync(qflag, 0, (uint64_t)nq * 4, c->stream))) return hip_err(c, e, "memset query flags");
+    GmArgs a{};
+    a.rp = (const uint64_t*)c->rp.ptr;
+    a.cols = (const uint32_t*)c->cols.ptr;
+    a.src = (const uint32_t*)c->src.ptr;
+    a.task_off = task_off;
+    a.ns = c->ns;
+    a.ntasks = ntasks;
+    a.chunk = cfg->window;
+    a.scratch = (uint32_t*)c->scratch.ptr;
+    a.cap = cap;
+    a.ctr = ctr;
+    a.per_root = nullptr;
+    GmBatchArgs b{};
+    b.progs = (const GmProgram*)c->batch_progs.ptr;
+    b.order = qord;
+    b.nq = nq;
+    b.levels = levels;
+    b.rpt = rpt;
+    b.per_q = per_q;
+    b.qcount = qcount;
+    b.nb = NodeBudget{qn, qflag, cfg->node_budget ? cfg->node_budget : ~0ull};
+    if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
+    hipLaunchKernelGGL(gm_batch_kernel, dim3(nblocks), dim3(bw * kWave), 0, c->stream, a, b);
+    if ((e = hipGetLastError())) return hip_err(c, e, "batch launch");
+    if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
+    std::vector<uint64_t> hq(2 * (uint64_t)nq);
+    std::vector<uint32_t> hf(nq);
+    unsigned long long h[GCTR_N];
+    if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
+    if ((e = hipMemcpyAsync(hq.data(), qcount, (uint64_t)nq * 16, hipMemcpyDeviceToHost, c->stream)))
+        return hip_err(c, e, "D2H query counts");
+    if ((e = hipMemcpyAsync(hf.data(), qflag, (uint64_t)nq * 4, hipMemcpyDeviceToHost, c->stream)))
+        return hip_err(c, e, "D2H query flags");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "batch sync");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    for (uint32_t q = 0; q < nq; q++) {
+        if (counts) counts[q] = hq[q];
+        if (qnodes) qnodes[q] = hq[nq + q];
+        if (over) over[q] = hf[q];
+    }
+    if (st) {
+        st->kernel_ms = ms;
+        st->launches = 1;
+        st->total_ms = ms;
+        st->nodes = h[GCTR_NODES];
         st->max_scratch_words = cap;
     }
     return G2_OK;

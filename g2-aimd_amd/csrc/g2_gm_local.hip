@@ -110,9 +110,7 @@ __global__ __launch_bounds__(256) void gm_local_kernel(LocalArgs a) {
     const LocalProgram& P = a.prog;
 
     while (true) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(&a.ctr[LCTR_NEXT], 1ull);
-        t = uniform((uint64_t)t);
+        const uint64_t t = fetch_task(&a.ctr[LCTR_NEXT]);
         if (t >= a.ntasks) break;
         const uint32_t v0 = uniform(a.list ? a.list[t] : (uint32_t)t);
         const uint64_t lo = uniform(a.rp[v0]), hi = uniform(a.rp[v0 + 1]);

@@ -88,6 +88,15 @@ struct g2_ctx {
     g2::LocalProgram lprog{};
     g2::DevBuf big_list2;
     g2_plan plan{};
+
+    // batched matching
+    std::vector<uint64_t> h_src_deg;  // host copy: degree of every source
+    g2::DevBuf batch_off;         // task offsets for a batch window != gm_task_chunk
+    uint32_t batch_window = 0;
+    uint64_t batch_ntasks = 0;
+    bool batch_off_valid = false;
+    g2::DevBuf batch_progs;       // GmProgram per query
+    g2::DevBuf batch_q;           // per-query counts, nodes, order, flags
 };
 
 namespace g2 {
@@ -103,5 +112,7 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_count,
 int gm_compile(g2_ctx* c, const g2_plan* p);
 void gm_local_compile(g2_ctx* c, const g2_plan* p);
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st);
+int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
+                 uint64_t* qnodes, uint32_t* over, g2_stats* st);
 
 }  // namespace g2

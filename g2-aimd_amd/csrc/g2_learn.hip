@@ -39,9 +39,7 @@ struct FeatArgs {
 __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
     const uint32_t lane = lane_id();
     while (true) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(a.next, 1ull);
-        t = uniform((uint64_t)t);
+        const uint64_t t = fetch_task(a.next);
         if (t >= a.ns) break;
         const uint32_t v = uniform(a.src[t]);
         const uint64_t lo = a.rp[v], hi = a.rp[v + 1], d = hi - lo;

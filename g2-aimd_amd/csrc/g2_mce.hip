@@ -84,9 +84,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
     uint64_t t_end = t_start;
 
     while (true) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(&a.ctr[CTR_NEXT], 1ull);
-        t = uniform((uint64_t)t) + a.first;
+        const uint64_t t = fetch_task(&a.ctr[CTR_NEXT]) + a.first;
         if (t >= a.ntasks) break;
         const uint64_t code = uniform(a.tasks ? a.tasks[t] : (uint64_t)t);
         const uint32_t si = (uint32_t)code;

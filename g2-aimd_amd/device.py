@@ -28,7 +28,7 @@ EXPORTS = ("g2_create", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce
            "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
            "g2_pinned_free", "g2_plan_local_eligible", "g2_features", "g2_mlp_create",
            "g2_mlp_destroy", "g2_mlp_get_params", "g2_mlp_set_params", "g2_mlp_infer", "g2_mlp_grad",
-           "g2_mlp_train")
+           "g2_mlp_train", "g2_gm_run_batch", "g2_gm_autotune")
 G2_NFEAT_INT = 5
 G2_MLP_NPARAM = 4801
 
@@ -51,6 +51,14 @@ class Stats(C.Structure):
 class Sched(C.Structure):
     _fields_ = [("order", C.c_void_p), ("n", C.c_uint64), ("begin", C.c_uint64),
                 ("end", C.c_uint64), ("split", C.c_void_p)]
+
+
+class Batch(C.Structure):
+    _fields_ = [("order", C.c_void_p), ("roots_per_task", C.c_uint32), ("window", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("reserved", C.c_uint32), ("node_budget", C.c_uint64)]
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("order", "reserved")}
 
 
 def hip_lib() -> C.CDLL:
@@ -78,6 +86,10 @@ def hip_lib() -> C.CDLL:
     L.g2_device_info.argtypes = [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.c_char_p, C.c_uint32]
     L.g2_plan_local_eligible.argtypes = [C.POINTER(G2Plan)]
+    L.g2_gm_run_batch.argtypes = [vp, C.POINTER(G2Plan), C.c_uint32, C.POINTER(Batch), u64p, u64p,
+                                  u32p, C.POINTER(Stats)]
+    L.g2_gm_autotune.argtypes = [vp, C.POINTER(G2Plan), C.c_uint32, C.c_uint32, C.c_uint64,
+                                 C.POINTER(Batch), C.POINTER(C.c_double)]
     f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.g2_features.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     L.g2_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
@@ -203,6 +215,47 @@ class Executor:
         out = {"count": cnt.value, "stats": st.to_dict()}
         if per_root:
             out["per_root_count"] = prc[: self.ns]
+        return out
+
+    @staticmethod
+    def _plans(plans):
+        from . import G2Plan
+        arr = (G2Plan * max(len(plans), 1))()
+        for i, p in enumerate(plans):
+            arr[i] = p
+        return arr
+
+    def run_gm_batch(self, plans, order=None, roots_per_task: int = 1, window: int = 0,
+                     waves_per_cu: int = 0, node_budget: int = 0) -> dict:
+        """Counts of many plans in one persistent launch (canonical traversal):
+        counts[q] equals run_gm(canonical=True) of plans[q] alone."""
+        nq = len(plans)
+        arr = self._plans(plans)
+        o = None
+        if order is not None:
+            o = np.ascontiguousarray(order, dtype=np.uint32)
+            if len(o) != nq:
+                raise ValueError("order must be a permutation of the queries")
+        b = Batch(o.ctypes.data if o is not None else None, roots_per_task, window, waves_per_cu, 0,
+                  node_budget)
+        counts = np.zeros(max(nq, 1), np.uint64)
+        nodes = np.zeros(max(nq, 1), np.uint64)
+        over = np.zeros(max(nq, 1), np.uint32)
+        st = Stats()
+        self._check(hip_lib().g2_gm_run_batch(self._ctx, arr, nq, C.byref(b), counts, nodes, over,
+                                              C.byref(st)), "g2_gm_run_batch")
+        return {"counts": counts[:nq], "nodes": nodes[:nq], "over_budget": over[:nq].astype(bool),
+                "stats": st.to_dict()}
+
+    def autotune(self, plans, sample_queries: int = 0, seed: int = 1, node_budget: int = 0) -> dict:
+        """Launch shape of run_gm_batch picked by timing a query sample."""
+        arr = self._plans(plans)
+        b = Batch(None, 0, 0, 0, 0, node_budget)
+        ms = C.c_double()
+        self._check(hip_lib().g2_gm_autotune(self._ctx, arr, len(plans), sample_queries, seed,
+                                             C.byref(b), C.byref(ms)), "g2_gm_autotune")
+        out = b.to_dict()
+        out["sample_ms"] = ms.value
         return out
 
     def features(self) -> np.ndarray:

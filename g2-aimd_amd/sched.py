@@ -1,0 +1,164 @@
+"""Learned root scheduling for maximal clique enumeration.
+
+The reference runs roots in source order, one warp per queued subgraph
+(app_BK/BK.h:323-355, system/pipeline_executor.h:151-207).  Here a root's
+whole subtree is one wave's work, so a few heavy roots can leave most waves
+idle at the end of a launch.  The scheduler
+  1. computes 8 features per root (GPU feature kernel + kcore shells),
+  2. predicts log(1 + subtree nodes) with the MFMA MLP cost model, trained on
+     the per-root node counts the plain enumerator logs,
+  3. orders roots by predicted cost (longest first) and deals the first-level
+     children of predicted-heavy roots over several waves.
+Neither step changes a result: the kernel's tree, counts, histogram and
+per-root node totals are the same for every order and split (tests).  The
+plain source order with no split is the fallback and the yardstick.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FEATURE_NAMES = ("log1p_degree", "log1p_core", "log1p_2paths", "log1p_triangles", "clustering",
+                 "degeneracy_rank", "shared_neighbour_fraction", "log1p_later_neighbours")
+
+
+def core_numbers(shells: np.ndarray, nv: int) -> np.ndarray:
+    """core number of every vertex of a kcore-recoded graph (shell index)"""
+    core = np.zeros(nv, np.int64)
+    for k in range(len(shells) - 1):
+        core[int(shells[k]):int(shells[k + 1])] = k
+    return core
+
+
+def root_features(ex, sources: np.ndarray, core: np.ndarray, nv: int) -> np.ndarray:
+    """float32 (ns, 8) model inputs of the uploaded sources"""
+    f = ex.features().astype(np.float64)
+    deg, two, tri, shared, p0 = (f[:, i] for i in range(5))
+    pairs = deg * (deg - 1)
+    clust = np.where(pairs > 0, 2 * tri / np.maximum(pairs, 1), 0.0)
+    src = np.asarray(sources, dtype=np.int64)
+    x = np.stack([np.log1p(deg), np.log1p(core[src]), np.log1p(two), np.log1p(tri), clust,
+                  src / max(nv, 1), shared / np.maximum(deg, 1), np.log1p(p0)], axis=1)
+    return x.astype(np.float32)
+
+
+def learned_schedule(pred_log_nodes: np.ndarray, later_neighbours: np.ndarray, waves: int,
+                     max_split: int = 4096):
+    """(order, split) from predicted log(1+nodes): longest predicted first;
+    a root predicted to exceed the per-wave share is dealt into
+    ceil(cost / share) parts (at most its number of later neighbours, the
+    most first-level children it can have)."""
+    cost = np.expm1(np.clip(pred_log_nodes.astype(np.float64), 0, 60))
+    order = np.argsort(-cost, kind="stable").astype(np.uint32)
+    share = max(cost.sum() / max(waves, 1), 1.0)
+    split = np.ceil(cost / share)
+    split = np.minimum(split, np.maximum(later_neighbours, 1))
+    split = np.clip(split, 1, max_split).astype(np.uint32)
+    return order, split
+
+
+def spearman(a: np.ndarray, b: np.ndarray) -> float:
+    ra = np.argsort(np.argsort(a, kind="stable"), kind="stable").astype(np.float64)
+    rb = np.argsort(np.argsort(b, kind="stable"), kind="stable").astype(np.float64)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    den = np.sqrt((ra * ra).sum() * (rb * rb).sum())
+    return float((ra * rb).sum() / den) if den > 0 else 0.0
+
+
+class Standardizer:
+    """per-feature affine map fitted on the training rows"""
+
+    def __init__(self, x: np.ndarray):
+        self.mu = x.mean(axis=0)
+        self.sd = np.maximum(x.std(axis=0), 1e-6)
+
+    def __call__(self, x: np.ndarray) -> np.ndarray:
+        return ((x - self.mu) / self.sd).astype(np.float32)
+
+
+def simulated_makespan(cost: np.ndarray, order: np.ndarray, split: np.ndarray, waves: int) -> float:
+    """greedy list schedule of per-task costs on `waves` identical workers in
+    the kernel's fetch order (a split root's cost is dealt evenly over its
+    parts): the makespan the dynamic fetch would reach if each wave's time
+    were proportional to its nodes"""
+    import heapq
+    load = [0.0] * waves
+    heapq.heapify(load)
+    for i in order:
+        k = int(split[i]) if split is not None else 1
+        part = float(cost[i]) / k
+        for _ in range(k):
+            t = heapq.heappop(load)
+            heapq.heappush(load, t + part)
+    return max(load)
+
+
+def prepare_graph(g2, scale: int, seed: int, edge_factor: int = 16):
+    """R-MAT graph of the matching workload family, degeneracy-ordered"""
+    g = g2.gen_rmat(scale, edge_factor, 0.57, 0.19, 0.19, seed).preprocess()
+    g, _, shells, _ = g.kcore_recode()
+    return g, core_numbers(shells, g.nv)
+
+
+def logged_rows(dev, g, core, **tuning):
+    """features and per-root log(1 + subtree nodes) logged by a plain run"""
+    with dev.Executor(device=0, **tuning) as ex:
+        ex.transfer(g)
+        x = root_features(ex, np.arange(g.nv, dtype=np.uint32), core, g.nv)
+        r = ex.run_mce(per_root=True)
+    return x, np.log1p(r["per_root_nodes"].astype(np.float64)).astype(np.float32), r
+
+
+def experiment(g2, dev, scale: int, train_seeds, test_seed: int, epochs: int = 30, batch: int = 512,
+               lr: float = 2e-3, reps: int = 3, log=print, **tuning) -> dict:
+    """Train the cost model on plain-run logs of the training seeds, schedule
+    the test seed's roots with it and compare against the plain order."""
+    xs, ys = [], []
+    for s in train_seeds:
+        g, core = prepare_graph(g2, scale, s)
+        x, y, _ = logged_rows(dev, g, core, **tuning)
+        xs.append(x)
+        ys.append(y)
+        log(f"train seed {s:#x}: {g.nv} roots, max log1p(nodes) {y.max():.2f}")
+    X, Y = np.concatenate(xs), np.concatenate(ys)
+    norm = Standardizer(X)
+    mlp = dev.Mlp(device=0, seed=1)
+    loss = mlp.train(norm(X), Y, epochs=epochs, batch=batch, lr=lr)
+    log(f"trained on {len(X)} roots: last-epoch mse {loss:.4f}")
+
+    g, core = prepare_graph(g2, scale, test_seed)
+    with dev.Executor(device=0, **tuning) as ex:
+        ex.transfer(g)
+        x = root_features(ex, np.arange(g.nv, dtype=np.uint32), core, g.nv)
+        pred = mlp.infer(norm(x))
+        plain = ex.run_mce(per_root=True)
+        truth = np.log1p(plain["per_root_nodes"].astype(np.float64))
+        info = dev.device_info(0)
+        waves = info["num_cus"] * max(1, tuning.get("waves_per_cu", 0) or 8)
+        later = ex.features()[:, 4]
+        order, split = learned_schedule(pred, later, waves)
+        learned = ex.run_mce(order=order, split=split)
+        if learned["total"] != plain["total"] or not np.array_equal(learned["hist"], plain["hist"]):
+            raise RuntimeError("learned schedule changed the result")
+        tp, tl = [], []
+        for _ in range(reps):
+            tp.append(ex.run_mce()["stats"])
+            tl.append(ex.run_mce(order=order, split=split)["stats"])
+    mlp.close()
+    nodes = plain["per_root_nodes"].astype(np.float64)
+    ident = np.arange(g.nv)
+    ms_p = float(np.median([s["kernel_ms"] for s in tp]))
+    ms_l = float(np.median([s["kernel_ms"] for s in tl]))
+    return {
+        "scale": scale, "train_roots": int(len(X)), "test_roots": int(g.nv),
+        "max_cliques": int(plain["total"]), "train_mse": loss,
+        "rmse_log_nodes": float(np.sqrt(np.mean((pred - truth) ** 2))),
+        "spearman": spearman(pred, truth),
+        "plain_ms": ms_p, "learned_ms": ms_l, "speedup": ms_p / ms_l,
+        "plain_idle_frac": float(np.median([s["wave_idle_frac"] for s in tp])),
+        "learned_idle_frac": float(np.median([s["wave_idle_frac"] for s in tl])),
+        "split_roots": int((split > 1).sum()), "split_parts": int(split.sum()),
+        "sim_makespan_plain": simulated_makespan(nodes, ident, None, waves),
+        "sim_makespan_learned": simulated_makespan(nodes, order, split, waves),
+        "sim_makespan_bound": max(nodes.sum() / waves, 1.0),
+    }

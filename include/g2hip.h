@@ -106,6 +106,41 @@ int  g2_gm_set_plan(g2_ctx* ctx, const g2_plan* plan);
 int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_count,
                g2_stats* stats);
 
+/* ---- batched matching (no counterpart in the reference: its run binary
+ * takes one query per process, app_gmatch/main.cu, system/worker.h:28-280) */
+
+/* launch shape of a batched run */
+typedef struct g2_batch {
+    const uint32_t* order;        /* processing order of the queries, a permutation of
+                                     [0, nq) (NULL = 0..nq-1) */
+    uint32_t roots_per_task;      /* root tasks a wave takes per fetch (0 = 1) */
+    uint32_t window;              /* level-1 window: candidates of a root row per root task
+                                     (0 = whole rows) */
+    uint32_t waves_per_cu;        /* resident waves per CU (0 = the context's tuning) */
+    uint32_t reserved;
+    uint64_t node_budget;         /* per-query limit on partial-match extensions; a query
+                                     that passes it is dropped and flagged (0 = none) */
+} g2_batch;
+
+/* Count nq plans over the uploaded view in one persistent launch.  counts[q]
+ * equals g2_gm_run's count of plans[q] alone (canonical traversal; the same
+ * root tasks summed into a per-query u64).  nodes (nullable) receives each
+ * query's partial-match extensions; over_budget (nullable) 1 where the query
+ * passed cfg->node_budget (its count is then incomplete).  cfg NULL = defaults.
+ * stats: kernel_ms of the launch, nodes over all queries. */
+int  g2_gm_run_batch(g2_ctx* ctx, const g2_plan* plans, uint32_t nq, const g2_batch* cfg,
+                     uint64_t* counts, uint64_t* nodes, uint32_t* over_budget, g2_stats* stats);
+
+/* Autotuner of the batched launch: times g2_gm_run_batch on a seeded sample
+ * of at most sample_queries plans (all when 0) by coordinate descent over the
+ * level-1 window, roots per task and resident waves per CU, and writes the
+ * fastest shape to *best.  best->node_budget is read first and bounds every
+ * trial; queries that pass it in the first trial are left out of the timing.
+ * best->order is set to NULL.  *best_ms (nullable) = the winner's kernel time
+ * on the sample. */
+int  g2_gm_autotune(g2_ctx* ctx, const g2_plan* plans, uint32_t nq, uint32_t sample_queries,
+                    uint64_t seed, g2_batch* best, double* best_ms);
+
 /* 1 if g2_gm_run counts this plan's whole-graph totals with the oriented
  * local-bitmap kernel: the root is a back neighbour of, and ordered below,
  * every other position, and the order conditions pick exactly one member of
