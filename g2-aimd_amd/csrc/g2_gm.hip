@@ -107,6 +107,18 @@ __device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn
         ohi[no] = h;
         no++;
     }
+    if (!out && no == 0) {
+        // counting with no membership test: the whole slice passes except the
+        // entries a NON_EQUAL condition names (matched values are distinct)
+        uint64_t cnt = s1 - s0;
+        for (uint32_t k = 0; k < ncond; k++) {
+            const uint32_t m = S.M[pos[k]];
+            if (op[k] == 2 && m >= rg.lo && m < rg.hi && contains(cols, s0, s1, m)) cnt--;
+        }
+        if (lane == 0) *lane_count += cnt;
+        if (ACCOUNT && npass_out) *npass_out = cnt;
+        return 0;
+    }
     uint64_t nout = 0, npass = 0;
     for (uint64_t base = s0; base < s1; base += kWave) {
         const uint64_t e = base + lane;
@@ -152,6 +164,15 @@ __device__ uint64_t filter_pre(const GmArgs& a, const WaveState& S, const GmStep
         if (ACCOUNT && lane == 0) *bytes += cost_isect(plen, ahi[k] - alo[k]);
         alo[k] = lower_bound(a.cols, alo[k], ahi[k], rg.lo);
         if (rg.hi != 0xffffffffu) ahi[k] = lower_bound(a.cols, alo[k], ahi[k], rg.hi);
+    }
+    if (!out && st.nafter == 0) {  // counting, no after rows: closed form as in filter_row
+        uint64_t cnt = p1 - p0;
+        for (uint32_t k = 0; k < st.nacond; k++) {
+            const uint32_t m = S.M[st.acpos[k]];
+            if (st.acop[k] == 2 && m >= rg.lo && m < rg.hi && contains(pre, p0, p1, m)) cnt--;
+        }
+        if (lane == 0) *lane_count += cnt;
+        return 0;
     }
     uint64_t nout = 0;
     for (uint64_t base = p0; base < p1; base += kWave) {

@@ -52,6 +52,7 @@ struct MceArgs {
     uint32_t hist_len;
     unsigned long long* per_root;        // nullable
     unsigned long long* wave_span;       // per wave: first task start, last task end (GPU real-time clock)
+    uint32_t small_max;                  // roots with d <= small_max (<= 64) take the register-resident path
 };
 
 // u64 words of the per-root arena: adjP (d*W) + adjX (p0*WX) + (p0+1) frames
@@ -70,16 +71,167 @@ __device__ inline void arena_sync() { wave_sync(); }
 
 __device__ inline uint64_t degree_of(const uint64_t* __restrict__ rp, uint32_t u) { return rp[u + 1] - rp[u]; }
 
+constexpr int kMaxBlockWaves = 4;  // __launch_bounds__(256)
+
+// LDS of one wave for the register-resident path
+struct SmallLds {
+    uint32_t C[kWave];         // the root's neighbourhood, sorted
+    uint32_t pref[kWave];      // exclusive prefix of the streamed row lengths per P0 vertex
+    uint64_t start[kWave];     // first streamed cols index per P0 vertex
+    uint64_t adjP[kWave];      // N(C[k]) & P0, bit j = P0[j]
+    uint64_t adjX[kWave];      // N(P0[j]) & X0, bit k = C[k]
+};
+
+__device__ inline uint32_t lds_lower_bound(const uint32_t* A, uint32_t lo, uint32_t hi, uint32_t val) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A[mid] < val) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t wave_max32(uint32_t x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t y = __shfl_xor(x, m, kWave);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// Roots with d <= 64: the whole subtree in registers.  Position k of C maps
+// to lane k; lane k keeps row adjP[k] and (k < p0) row adjX[k]; a node is four
+// wave-uniform words P, XP (P0 space), XX (X0 space) and Q; the DFS stack is
+// spread over the lanes (lane i holds the frame of depth i).  The tree, the
+// pivot rule (max |P & N(u)|, ties to the larger position) and the child order
+// are those of the arena path, so every count is unchanged.  Clique sizes are
+// counted per lane (lane = size, sizes 64 and 65 in hbig at lanes 0 and 1) and
+// flushed once per wave.
+__device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t v, uint64_t lo, uint32_t d, uint32_t s,
+                                      uint32_t part, uint32_t parts, uint64_t& cliques, uint64_t& hcount,
+                                      uint64_t& hbig) {
+    const uint32_t lane = lane_id();
+    const uint32_t p0 = d - s;
+    // ---- neighbourhood bitsets in LDS ----
+    if (lane < d) L.C[lane] = a.cols[lo + lane];
+    L.adjP[lane] = 0;
+    L.adjX[lane] = 0;
+    wave_sync();
+    uint32_t len = 0;
+    if (lane < p0) {
+        const uint32_t p = L.C[s + lane];
+        uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
+        if (e1 - e0 > kWave) {  // restrict a long row to [C[0], C[d-1]]
+            e0 = lower_bound(a.cols, e0, e1, L.C[0]);
+            e1 = lower_bound(a.cols, e0, e1, L.C[d - 1] + 1);
+        }
+        L.start[lane] = e0;
+        len = (uint32_t)(e1 - e0);
+    }
+    uint32_t incl = len;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    L.pref[lane] = incl - len;
+    const uint32_t T = read_lane(incl, kWave - 1);
+    wave_sync();
+    for (uint32_t f = lane; f < T; f += kWave) {
+        // P0 vertex j whose streamed slice holds entry f (last j with pref[j] <= f)
+        uint32_t lo2 = 0, hi2 = p0;
+        while (hi2 - lo2 > 1) {
+            const uint32_t mid = (lo2 + hi2) >> 1;
+            if (L.pref[mid] <= f) lo2 = mid; else hi2 = mid;
+        }
+        const uint32_t j = lo2;
+        const uint32_t w = a.cols[L.start[j] + (f - L.pref[j])];
+        const uint32_t k = lds_lower_bound(L.C, 0, d, w);
+        if (k < d && L.C[k] == w) {
+            if (k < s) {
+                atomicOr((unsigned long long*)&L.adjX[j], 1ull << k);
+                atomicOr((unsigned long long*)&L.adjP[k], 1ull << j);
+            } else {
+                atomicOr((unsigned long long*)&L.adjP[s + j], 1ull << (k - s));
+            }
+        }
+    }
+    wave_sync();
+    const uint64_t rowP = L.adjP[lane];
+    const uint64_t rowX = L.adjX[lane];
+
+    // ---- depth-first walk ----
+    uint64_t P = ones(p0), XP = 0, XX = ones(s), Q = 0;
+    uint64_t fP = 0, fXP = 0, fXX = 0, fQ = 0;  // frame of depth == lane
+    uint32_t depth = 0, child = 0;
+    uint64_t nodes = 0;
+    bool fresh = true;
+    while (true) {
+        bool descend = false;
+        if (fresh) {
+            if (depth > 0 || part == 0) nodes++;
+            if (P == 0) {
+                if (XP == 0 && XX == 0) {  // maximal clique, |R| = depth + 1
+                    cliques++;
+                    const uint32_t sz = depth + 1;
+                    if (sz < kWave) hcount += (lane == sz);
+                    else hbig += (lane == sz - kWave);
+                }
+            } else {
+                const uint64_t PX = P | XP;
+                const bool cand = lane < s ? ((XX >> lane) & 1) : (lane < d && ((PX >> (lane - s)) & 1));
+                const uint32_t key = cand ? ((uint32_t)__popcll(rowP & P) << 8) | lane : 0u;
+                const uint32_t pivot = wave_max32(key) & 0xffu;
+                Q = P & ~read_lane(rowP, pivot);
+            }
+            fresh = false;
+        }
+        if (Q != 0) {
+            const uint32_t j = __builtin_ctzll(Q);
+            const uint64_t m = 1ull << j;
+            Q &= ~m;
+            P &= ~m;
+            XP |= m;
+            if (depth == 0 && (child++ % parts) != part) continue;  // another part's child
+            const bool mine = lane == depth;
+            fP = mine ? P : fP;
+            fXP = mine ? XP : fXP;
+            fXX = mine ? XX : fXX;
+            fQ = mine ? Q : fQ;
+            const uint64_t qrow = read_lane(rowP, s + j);
+            P &= qrow;
+            XP &= qrow;
+            XX &= read_lane(rowX, j);
+            Q = 0;
+            depth++;
+            fresh = true;
+            descend = true;
+        }
+        if (!descend) {
+            if (depth == 0) break;
+            depth--;
+            P = read_lane(fP, depth);
+            XP = read_lane(fXP, depth);
+            XX = read_lane(fXX, depth);
+            Q = read_lane(fQ, depth);
+        }
+    }
+    return nodes;
+}
+
 template <bool ACCOUNT>
 __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
     __shared__ unsigned long long lhist[64];
+    __shared__ SmallLds small_lds[kMaxBlockWaves];
     for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x) lhist[i] = 0;
     __syncthreads();
 
     const uint32_t lane = lane_id();
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    SmallLds& SL = small_lds[threadIdx.x / kWave];
     uint64_t* ws = a.scratch + gwave * a.scratch_words;
     uint64_t my_cliques = 0, my_nodes = 0, my_bytes = 0;
+    uint64_t small_cliques = 0, small_hist = 0, small_big = 0;  // register-resident path tallies
     const uint64_t t_start = wall_clock64();
     uint64_t t_end = t_start;
 
@@ -105,6 +257,14 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                 my_nodes += 1;
                 if (a.per_root && lane == 0) atomicAdd(&a.per_root[si], 1ull);
             }
+            continue;
+        }
+        if (!ACCOUNT && d <= a.small_max) {
+            root_nodes = small_root(a, SL, v, lo, (uint32_t)d, (uint32_t)s, part, parts, small_cliques, small_hist,
+                                    small_big);
+            my_nodes += root_nodes;
+            if (a.per_root && lane == 0 && root_nodes) atomicAdd(&a.per_root[si], (unsigned long long)root_nodes);
+            t_end = wall_clock64();
             continue;
         }
         const uint64_t W = (p0 + 63) / 64, WX = (s + 63) / 64, FW = 3 * W + WX;
@@ -323,6 +483,13 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
 
     // ---- flush per-wave and per-block tallies ----------------------------
     if (ACCOUNT) my_bytes = wave_sum(my_bytes);
+    if (!ACCOUNT) {
+        // small-root tallies: every lane holds the same clique count; lane i
+        // holds the cliques of size i
+        my_cliques += small_cliques;
+        if (small_hist) atomicAdd(&lhist[lane], (unsigned long long)small_hist);
+        if (small_big) atomicAdd(&a.hist[min<uint32_t>(kWave + lane, a.hist_len - 1)], (unsigned long long)small_big);
+    }
     if (lane == 0) {
         if (my_cliques) atomicAdd(&a.ctr[CTR_CLIQUES], (unsigned long long)my_cliques);
         if (my_nodes) atomicAdd(&a.ctr[CTR_NODES], (unsigned long long)my_nodes);
@@ -337,7 +504,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
             uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
     const uint32_t hl = hist ? hist_len : 2;
-    const uint32_t bw = c->tune.block_waves;
+    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
     const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
     const uint64_t nwaves = nblocks * bw;
     const uint64_t sw = c->tune.wave_scratch_words;
@@ -390,6 +557,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     a.hist_len = hl;
     a.per_root = per_root_nodes ? (unsigned long long*)c->per_root.ptr : nullptr;
     a.wave_span = (unsigned long long*)c->spans.ptr;
+    a.small_max = std::min<uint32_t>(c->tune.mce_small_max, kWave);
 
     auto launch = [&](uint64_t blocks) {
         if (account) hipLaunchKernelGGL(mce_kernel<true>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);

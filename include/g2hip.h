@@ -49,6 +49,8 @@ typedef struct g2_tuning {
     uint64_t wave_scratch_words;  /* per-wave u64 words for MCE local bitsets (small-root pass) */
     uint32_t big_waves;           /* waves of the large-root pass */
     uint32_t gm_chunk;            /* level-1 candidates per matching task (0 = whole root) */
+    uint32_t mce_small_max;       /* MCE roots of at most this degree run in registers
+                                     (0 = 64, the most; 1 = practically off) */
 } g2_tuning;
 
 /* measurements of the last run */

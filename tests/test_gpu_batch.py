@@ -41,36 +41,49 @@ def test_batch_equals_solo_and_oracle(g2, orc, gpu_exec, B, rmat9):
     assert r["stats"]["nodes"] == int(r["nodes"].sum())
 
 
+@pytest.fixture(scope="module")
+def gnp80(g2):
+    import graphs as G
+    return G.gnp(g2, 80, 0.12, 5)
+
+
+@pytest.fixture(scope="module")
+def solo_exec(g2):
+    """solo runs with level-1 windows of 16, so hub roots spread over waves"""
+    ex = g2.Executor(device=0, gm_chunk=16)
+    yield ex
+    ex.close()
+
+
 @pytest.mark.parametrize("shape", [dict(window=0, roots_per_task=1), dict(window=8, roots_per_task=1),
                                    dict(window=32, roots_per_task=8), dict(window=0, roots_per_task=64,
                                                                            waves_per_cu=2)])
-def test_batch_shapes_and_order(g2, gpu_exec, B, rmat9, shape):
+def test_batch_shapes_and_order(g2, orc, gpu_exec, B, gnp80, shape):
+    """5-7 vertex patterns: every launch shape and query order gives the oracle's counts"""
     print("case: batch shape", shape, flush=True)
-    qs = B.random_queries(g2, 64, SEED + 1, nmax=6)
+    qs = B.random_queries(g2, 64, SEED + 1, nmin=5, nmax=7)
     plans = [g2.plan_build(q) for q in qs]
-    gpu_exec.transfer(rmat9)
-    base = gpu_exec.run_gm_batch(plans)
-    order = B.heavy_first(qs)
-    r = gpu_exec.run_gm_batch(plans, order=order, **shape)
-    assert np.array_equal(r["counts"], base["counts"])
-    assert np.array_equal(base["counts"], solo(gpu_exec, plans))
+    gpu_exec.transfer(gnp80)
+    r = gpu_exec.run_gm_batch(plans, order=B.heavy_first(qs), **shape)
+    ref = np.array([orc.gm(gnp80, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"] for p in plans], np.uint64)
+    assert np.array_equal(r["counts"], ref)
 
 
-def test_batch_node_budget(g2, gpu_exec, B, rmat9):
+def test_batch_node_budget(g2, solo_exec, B, rmat9):
     """Queries over the node budget are flagged (their charged nodes exceed it);
     every unflagged count equals the solo run, and a larger budget keeps them."""
     print("case: batch node budget", flush=True)
     qs = B.random_queries(g2, 64, SEED + 2, nmin=4, nmax=7)
     plans = [g2.plan_build(q) for q in qs]
-    gpu_exec.transfer(rmat9)
-    limit = 2_000_000
-    r1 = gpu_exec.run_gm_batch(plans, node_budget=limit)
+    solo_exec.transfer(rmat9)
+    limit = 200_000
+    r1 = solo_exec.run_gm_batch(plans, node_budget=limit, window=16)
     over = r1["over_budget"]
     assert over.any() and not over.all()
     assert (r1["nodes"][over] > limit).all() and (r1["nodes"][~over] <= limit).all()
     keep = [p for p, o in zip(plans, over) if not o]
-    assert np.array_equal(r1["counts"][~over], solo(gpu_exec, keep))
-    r2 = gpu_exec.run_gm_batch(plans, node_budget=4 * limit, window=16, roots_per_task=4)
+    assert np.array_equal(r1["counts"][~over], solo(solo_exec, keep))
+    r2 = solo_exec.run_gm_batch(plans, node_budget=4 * limit, window=32, roots_per_task=4)
     assert not (r2["over_budget"] & ~over).any()
     assert np.array_equal(r2["counts"][~over], r1["counts"][~over])
 
@@ -90,7 +103,7 @@ def test_batch_subset_sources_and_empty(g2, orc, gpu_exec, B, rmat9):
 
 def test_autotune(g2, gpu_exec, B, rmat9):
     print("case: autotune", flush=True)
-    qs = B.random_queries(g2, 40, SEED + 4, nmax=6)
+    qs = B.random_queries(g2, 40, SEED + 4, nmax=5)
     plans = [g2.plan_build(q) for q in qs]
     gpu_exec.transfer(rmat9)
     t = gpu_exec.autotune(plans, sample_queries=16, seed=3, node_budget=1 << 40)

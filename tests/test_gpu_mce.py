@@ -58,6 +58,20 @@ def test_chung_lu_100k(g2, orc, gpu_exec, cfg1):
     g, _ = cfg1
     got, ref = check(gpu_exec, orc, g, account=True)
     assert got["stats"]["alg_bytes"] == ref["bytes"]
+    check(gpu_exec, orc, g)  # register-resident path for roots of degree <= 64
+
+
+@pytest.mark.parametrize("small_max", [1, 8, 33, 64])
+def test_register_path_bounds(g2, orc, small_max):
+    """Roots of degree <= small_max run in registers, the rest in the arena:
+    every split point gives the oracle's results (degrees up to ~100 here)."""
+    print("case: register path bound", small_max, flush=True)
+    with g2.Executor(device=0, mce_small_max=small_max) as ex:
+        for seed, (n, p) in enumerate([(120, 0.52), (300, 0.08)]):
+            g, _, _, _ = G.gnp(g2, n, p, 40 + seed).kcore_recode()
+            check(ex, orc, g)
+        check(ex, orc, G.complete(g2, 66))  # d = 65: the arena path; a 66-clique
+        check(ex, orc, G.complete(g2, 64))  # d = 63 everywhere
 
 
 def test_chung_lu_views(g2, orc, gpu_exec, cfg1):
