@@ -97,11 +97,11 @@ int g2_create(g2_ctx** out, int device, uint64_t budget, const g2_tuning* t) {
     if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return G2_EHIP; }
     c->num_cus = p.multiProcessorCount;
     if (t) c->tune = *t;
-    if (!c->tune.waves_per_cu) c->tune.waves_per_cu = 8;
     if (!c->tune.block_waves) c->tune.block_waves = 4;
     if (!c->tune.wave_scratch_words) c->tune.wave_scratch_words = 1ull << 16;
     if (!c->tune.big_waves) c->tune.big_waves = 256;
     if (!c->tune.mce_small_max) c->tune.mce_small_max = 64;
+    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 32;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return G2_EHIP; }
     for (auto& ev : c->ev)
         if (hipEventCreate(&ev) != hipSuccess) { g2_destroy(c); return G2_EHIP; }
@@ -115,7 +115,7 @@ void g2_destroy(g2_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
                       &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans,
-                      &c->batch_off, &c->batch_progs, &c->batch_q})
+                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);

@@ -532,7 +532,7 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_st
     if (c->local_ok && c->sources_all && !per_root && !account && !(flags & G2_RUN_CANONICAL))
         return gm_local_run(c, count, st);
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
-    uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
+    uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM) / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
     const uint64_t per_wave = cap * 2 * c->prog.n * 4;
     // shrink the grid until the arenas fit the budget
@@ -645,7 +645,7 @@ int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* c
     if ((rc = batch_task_offsets(c, cfg->window, &task_off, &ntasks))) return rc;
     const uint32_t rpt = std::max<uint32_t>(cfg->roots_per_task, 1);
     const uint64_t per_q = (ntasks + rpt - 1) / rpt;
-    const uint32_t wpc = cfg->waves_per_cu ? cfg->waves_per_cu : c->tune.waves_per_cu;
+    const uint32_t wpc = cfg->waves_per_cu ? cfg->waves_per_cu : waves_per_cu(c, WK_GM);
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * wpc / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);

@@ -348,7 +348,7 @@ static void launch_local(uint64_t blocks, uint32_t bw, hipStream_t s, const Loca
 
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     const uint32_t bw = c->tune.block_waves;
-    const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
+    const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM_LOCAL) / bw);
     const uint64_t nwaves = nblocks * bw;
     const uint64_t sw = c->tune.wave_scratch_words;
     int rc;

@@ -80,6 +80,7 @@ struct g2_ctx {
     g2::DevBuf per_root;
     g2::DevBuf order;
     g2::DevBuf spans;
+    g2::DevBuf donate;            // MCE donation queue (counters, record table, pool)
 
     // matching
     bool has_plan = false;
@@ -102,6 +103,14 @@ struct g2_ctx {
 namespace g2 {
 
 int set_err(g2_ctx* c, int code, const std::string& msg);
+
+// resident waves per CU of a persistent grid: the context's setting, else the
+// kernel's own default (measured on MI355X: latency-bound searches gain up to
+// the occupancy their register use allows)
+enum WaveKind { WK_MCE = 16, WK_GM = 16, WK_GM_LOCAL = 32, WK_FEATURES = 16 };
+inline uint32_t waves_per_cu(const g2_ctx* c, WaveKind k) {
+    return c->tune.waves_per_cu ? c->tune.waves_per_cu : (uint32_t)k;
+}
 int hip_err(g2_ctx* c, hipError_t e, const char* what);
 // make sure buf holds at least `bytes`; charged against the budget
 int ensure(g2_ctx* c, DevBuf& buf, uint64_t bytes, const char* what);

@@ -84,7 +84,7 @@ int features_run(g2_ctx* c, int64_t* out) {
     if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
     FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
                (unsigned long long*)c->counters.ptr, (int64_t*)dout.ptr};
-    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / 4);
+    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
     hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
     if ((e = hipGetLastError())) return hip_err(c, e, "feature launch");
     if (c->ns && (e = hipMemcpyAsync(out, dout.ptr, c->ns * G2_NFEAT_INT * 8, hipMemcpyDeviceToHost, c->stream)))

@@ -53,7 +53,22 @@ struct MceArgs {
     unsigned long long* per_root;        // nullable
     unsigned long long* wave_span;       // per wave: first task start, last task end (GPU real-time clock)
     uint32_t small_max;                  // roots with d <= small_max (<= 64) take the register-resident path
+    // work donation: a wave that sees idle waves hands its shallowest open
+    // frame (its untried children) to a queue the idle waves drain
+    unsigned long long* dq;              // DQ_* counters
+    uint64_t* pool;                      // record words
+    uint64_t pool_words;
+    unsigned long long* rec_off;         // per record: offset in pool
+    unsigned int* rec_ready;             // per record: published flag
+    uint64_t max_records;
+    uint32_t donate_every;               // nodes between checks for idle waves (0 = never donate)
 };
+
+// donation queue counters: records reserved, records taken, root tasks
+// finished (flushed per wave), waves holding a record, waves waiting for one,
+// pool words handed out, records published
+enum { DQ_RES = 0, DQ_TAKEN, DQ_DONE, DQ_BUSY, DQ_IDLE, DQ_TOP, DQ_MADE, DQ_N };
+constexpr uint64_t kDqStride = 16;  // one 128-byte line per counter
 
 // u64 words of the per-root arena: adjP (d*W) + adjX (p0*WX) + (p0+1) frames
 __host__ __device__ inline uint64_t mce_need(uint64_t d, uint64_t s) {
@@ -70,6 +85,109 @@ __device__ inline uint64_t ones(uint64_t nbits) { return nbits >= 64 ? ~0ull : (
 __device__ inline void arena_sync() { wave_sync(); }
 
 __device__ inline uint64_t degree_of(const uint64_t* __restrict__ rp, uint32_t u) { return rp[u + 1] - rp[u]; }
+
+__device__ inline unsigned long long ld_agent(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Donation queue.  A record is [source index, absolute depth, P (W words),
+// XP (W), Q (W), XX (WX)]: a frame whose untried children Q the taker walks
+// exactly as the donor would have (same P/XP progression, same order), so
+// the search tree and every count stay the same.
+//
+// Termination.  A wave adds the number of root tasks it finished to DQ_DONE
+// when it leaves the root phase (a wave that never started adds nothing and
+// is never waited for), and DQ_BUSY counts waves holding a record.  Records
+// are published only by waves inside a root task or holding a record.  An
+// idle wave exits once every root task is done, no wave holds a record and
+// every reserved record has been taken.  A wave that published records
+// claims any still unclaimed before it can exit, so none is left behind.
+// Only these per-wave transitions and the (rare) donations use acquire /
+// release at device scope; the per-root fetch stays a relaxed atomic.
+
+// true when idle waves outnumber the records waiting for them
+__device__ inline bool hungry(const MceArgs& a) {
+    __builtin_amdgcn_wave_barrier();
+    uint32_t h = 0;
+    if (lane_id() == 0) {
+        const uint64_t idle = ld_agent(&a.dq[DQ_IDLE * kDqStride]);
+        const uint64_t res = ld_agent(&a.dq[DQ_RES * kDqStride]), taken = ld_agent(&a.dq[DQ_TAKEN * kDqStride]);
+        h = idle > res - taken;
+    }
+    return uniform(h) != 0;
+}
+
+// reserve a record of `words` words; false when the pool or the table is full
+__device__ inline bool reserve_record(const MceArgs& a, uint64_t words, uint64_t* idx, uint64_t* off) {
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long i = ~0ull, o = 0;
+    if (lane_id() == 0) {
+        o = atomicAdd(&a.dq[DQ_TOP * kDqStride], (unsigned long long)words);
+        if (o + words <= a.pool_words) {
+            unsigned long long c = ld_agent(&a.dq[DQ_RES * kDqStride]);
+            while (c < a.max_records) {
+                const unsigned long long prev = atomicCAS(&a.dq[DQ_RES * kDqStride], c, c + 1);
+                if (prev == c) { i = c; break; }
+                c = prev;
+            }
+        }
+    }
+    *idx = uniform((uint64_t)i);
+    *off = uniform((uint64_t)o);
+    return *idx != ~0ull;
+}
+
+// the record words are written by the whole wave: make them visible, then flag
+__device__ inline void publish_record(const MceArgs& a, uint64_t idx, uint64_t off) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    if (lane_id() == 0) {
+        __hip_atomic_store(&a.rec_off[idx], (unsigned long long)off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(&a.dq[DQ_MADE * kDqStride], 1ull);
+    }
+}
+
+// add (wrapping) to a queue counter with release order: after the wave's publishes
+__device__ inline void dq_add(const MceArgs& a, int which, unsigned long long v) {
+    __builtin_amdgcn_wave_barrier();
+    if (lane_id() == 0)
+        __hip_atomic_fetch_add(&a.dq[which * kDqStride], v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Next record for an idle wave (counted in DQ_IDLE): its pool offset, or ~0
+// when the run is over.  A claimed record moves the wave from idle to busy.
+__device__ inline uint64_t claim_record(const MceArgs& a, uint64_t total_tasks) {
+    while (true) {
+        __builtin_amdgcn_wave_barrier();
+        unsigned long long got = ~0ull - 1;  // nothing yet
+        if (lane_id() == 0) {
+            const unsigned long long taken = ld_agent(&a.dq[DQ_TAKEN * kDqStride]);
+            const unsigned long long res = ld_agent(&a.dq[DQ_RES * kDqStride]);
+            if (taken < res) {
+                if (atomicCAS(&a.dq[DQ_TAKEN * kDqStride], taken, taken + 1) == taken) {
+                    atomicAdd(&a.dq[DQ_BUSY * kDqStride], 1ull);
+                    atomicAdd(&a.dq[DQ_IDLE * kDqStride], ~0ull);
+                    // the publisher is inside a task, so the flag comes
+                    while (__hip_atomic_load(&a.rec_ready[taken], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                        __builtin_amdgcn_s_sleep(1);
+                    got = __hip_atomic_load(&a.rec_off[taken], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            } else if (__hip_atomic_load(&a.dq[DQ_DONE * kDqStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == total_tasks &&
+                       __hip_atomic_load(&a.dq[DQ_BUSY * kDqStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                       ld_agent(&a.dq[DQ_TAKEN * kDqStride]) == ld_agent(&a.dq[DQ_RES * kDqStride])) {
+                atomicAdd(&a.dq[DQ_IDLE * kDqStride], ~0ull);
+                got = ~0ull;
+            }
+        }
+        got = uniform((uint64_t)got);
+        if (got != ~0ull - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return got;
+        }
+        __builtin_amdgcn_s_sleep(32);
+    }
+}
 
 constexpr int kMaxBlockWaves = 4;  // __launch_bounds__(256)
 
@@ -107,9 +225,13 @@ __device__ inline uint32_t wave_max32(uint32_t x) {
 // are those of the arena path, so every count is unchanged.  Clique sizes are
 // counted per lane (lane = size, sizes 64 and 65 in hbig at lanes 0 and 1) and
 // flushed once per wave.
-__device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t v, uint64_t lo, uint32_t d, uint32_t s,
-                                      uint32_t part, uint32_t parts, uint64_t& cliques, uint64_t& hcount,
-                                      uint64_t& hbig) {
+//
+// With rec != nullptr the walk resumes a donated frame (W = WX = 1 here):
+// it starts at that node with its untried children, one level above which
+// the walk ends.
+__device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si, uint32_t v, uint64_t lo, uint32_t d,
+                                      uint32_t s, uint32_t part, uint32_t parts, const uint64_t* rec,
+                                      uint64_t& cliques, uint64_t& hcount, uint64_t& hbig) {
     const uint32_t lane = lane_id();
     const uint32_t p0 = d - s;
     // ---- neighbourhood bitsets in LDS ----
@@ -160,20 +282,49 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t v,
     const uint64_t rowP = L.adjP[lane];
     const uint64_t rowX = L.adjX[lane];
 
-    // ---- depth-first walk ----
+    // ---- depth-first walk (depth relative to `base`, the start node's depth) ----
     uint64_t P = ones(p0), XP = 0, XX = ones(s), Q = 0;
-    uint64_t fP = 0, fXP = 0, fXX = 0, fQ = 0;  // frame of depth == lane
-    uint32_t depth = 0, child = 0;
-    uint64_t nodes = 0;
+    uint32_t base = 0;
     bool fresh = true;
+    if (rec) {
+        base = (uint32_t)uniform(rec[1]);
+        P = uniform(rec[2]);
+        XP = uniform(rec[3]);
+        Q = uniform(rec[4]);
+        XX = uniform(rec[5]);
+        fresh = false;
+    }
+    uint64_t fP = 0, fXP = 0, fXX = 0, fQ = 0;  // frame of depth == lane
+    uint32_t depth = 0, child = 0, since = 0;
+    uint64_t nodes = 0;
     while (true) {
         bool descend = false;
         if (fresh) {
-            if (depth > 0 || part == 0) nodes++;
+            if (base + depth > 0 || part == 0) nodes++;
+            if (a.donate_every && ++since >= a.donate_every) {
+                since = 0;
+                // hand the shallowest frame with untried children to an idle wave
+                // (not the root frame of a split root: its children are dealt by part)
+                const uint32_t first = (base == 0 && parts > 1) ? 1u : 0u;
+                const uint64_t open = ballot(lane < depth && lane >= first && fQ != 0);
+                if (open && hungry(a)) {
+                    const uint32_t i = __builtin_ctzll(open);
+                    uint64_t idx, off;
+                    if (reserve_record(a, 6, &idx, &off)) {
+                        const uint64_t fp = read_lane(fP, i), fxp = read_lane(fXP, i), fq = read_lane(fQ, i),
+                                       fxx = read_lane(fXX, i);
+                        const uint64_t w = lane == 0 ? (uint64_t)si : lane == 1 ? (uint64_t)(base + i)
+                                         : lane == 2 ? fp : lane == 3 ? fxp : lane == 4 ? fq : fxx;
+                        if (lane < 6) a.pool[off + lane] = w;
+                        publish_record(a, idx, off);
+                        fQ = lane == i ? 0 : fQ;
+                    }
+                }
+            }
             if (P == 0) {
-                if (XP == 0 && XX == 0) {  // maximal clique, |R| = depth + 1
+                if (XP == 0 && XX == 0) {  // maximal clique, |R| = base + depth + 1
                     cliques++;
-                    const uint32_t sz = depth + 1;
+                    const uint32_t sz = base + depth + 1;
                     if (sz < kWave) hcount += (lane == sz);
                     else hbig += (lane == sz - kWave);
                 }
@@ -192,7 +343,7 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t v,
             Q &= ~m;
             P &= ~m;
             XP |= m;
-            if (depth == 0 && (child++ % parts) != part) continue;  // another part's child
+            if (base + depth == 0 && (child++ % parts) != part) continue;  // another part's child
             const bool mine = lane == depth;
             fP = mine ? P : fP;
             fXP = mine ? XP : fXP;
@@ -235,10 +386,8 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
     const uint64_t t_start = wall_clock64();
     uint64_t t_end = t_start;
 
-    while (true) {
-        const uint64_t t = fetch_task(&a.ctr[CTR_NEXT]) + a.first;
-        if (t >= a.ntasks) break;
-        const uint64_t code = uniform(a.tasks ? a.tasks[t] : (uint64_t)t);
+    // one task: a root (part `part` of `parts`), or a donated frame when rec != nullptr
+    auto process = [&](uint64_t code, const uint64_t* rec) {
         const uint32_t si = (uint32_t)code;
         const uint32_t part = (uint32_t)(code >> 32) & 0xffffu;
         const uint32_t parts = (uint32_t)(code >> 48) ? (uint32_t)(code >> 48) : 1u;
@@ -248,30 +397,30 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         const uint32_t* __restrict__ C = a.cols + lo;
         uint64_t root_nodes = 0;
 
-        if (d == 0) continue;  // no subgraph is generated for an isolated root
+        if (d == 0) return;  // no subgraph is generated for an isolated root
         const uint64_t s = uniform(lower_bound(C, 0, d, v));
         const uint64_t p0 = d - s;
-        if (p0 == 0) {  // root node with empty P and non-empty X: one node, no clique
+        if (p0 == 0) {  // root node with empty P and non-empty X: one node, no clique (never donated)
             if (part == 0) {
                 if (ACCOUNT && lane == 0) my_bytes += 5 * (1 + d);
                 my_nodes += 1;
                 if (a.per_root && lane == 0) atomicAdd(&a.per_root[si], 1ull);
             }
-            continue;
+            return;
         }
         if (!ACCOUNT && d <= a.small_max) {
-            root_nodes = small_root(a, SL, v, lo, (uint32_t)d, (uint32_t)s, part, parts, small_cliques, small_hist,
-                                    small_big);
+            root_nodes = small_root(a, SL, si, v, lo, (uint32_t)d, (uint32_t)s, part, parts, rec, small_cliques,
+                                    small_hist, small_big);
             my_nodes += root_nodes;
             if (a.per_root && lane == 0 && root_nodes) atomicAdd(&a.per_root[si], (unsigned long long)root_nodes);
             t_end = wall_clock64();
-            continue;
+            return;
         }
         const uint64_t W = (p0 + 63) / 64, WX = (s + 63) / 64, FW = 3 * W + WX;
         const uint64_t need = mce_need(d, s);
         if (W > kWave) {  // P sets wider than 64 words per lane are not supported
             if (lane == 0) atomicMax(&a.ctr[CTR_ERR], (unsigned long long)need);
-            continue;
+            return;
         }
         if (need > a.scratch_words) {
             if (lane == 0) {
@@ -283,7 +432,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                     atomicMax(&a.ctr[CTR_ERR], (unsigned long long)need);
                 }
             }
-            continue;
+            return;
         }
         uint64_t* adjP = ws;                 // d rows of W words
         uint64_t* adjX = ws + d * W;         // p0 rows of WX words
@@ -339,19 +488,52 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
 
         // ---- depth-first walk of the root's subtree -----------------------
         uint64_t pw = 0, xpw = 0, qw = 0;
-        if (lane < W) pw = ones(p0 - (uint64_t)lane * 64);
-        for (uint64_t i = lane; i < WX; i += kWave) frames[3 * W + i] = ones(s - i * 64);
-        uint64_t depth = 0;
+        uint64_t base = 0;   // depth of the start node (a donated frame's depth)
         bool fresh = true;
+        if (rec) {
+            base = uniform(rec[1]);
+            for (uint64_t i = lane; i < FW; i += kWave) frames[i] = rec[2 + i];
+            arena_sync();
+            if (lane < W) {
+                pw = frames[lane];
+                xpw = frames[W + lane];
+                qw = frames[2 * W + lane];
+            }
+            fresh = false;
+        } else {
+            if (lane < W) pw = ones(p0 - (uint64_t)lane * 64);
+            for (uint64_t i = lane; i < WX; i += kWave) frames[3 * W + i] = ones(s - i * 64);
+        }
+        uint64_t depth = 0;  // relative to base
         uint32_t child = 0;  // first-level children passed so far
+        uint32_t since = 0;  // nodes since the last look for idle waves
         while (true) {
             uint64_t* F = frames + depth * FW;
             uint64_t* XX = F + 3 * W;
             bool descend = false;
             // the root node belongs to part 0 of a split root
-            const bool own = depth > 0 || part == 0;
+            const bool own = base + depth > 0 || part == 0;
             if (fresh) {
                 if (own) root_nodes++;
+                if (!ACCOUNT && a.donate_every && ++since >= a.donate_every) {
+                    since = 0;
+                    if (hungry(a)) {
+                        // hand the shallowest ancestor frame with untried children to an
+                        // idle wave (not the root frame of a split root: dealt by part)
+                        for (uint64_t i = (base == 0 && parts > 1) ? 1 : 0; i < depth; i++) {
+                            const uint64_t* Fi = frames + i * FW;
+                            if (!ballot(lane < W && Fi[2 * W + lane] != 0)) continue;
+                            uint64_t idx, off;
+                            if (reserve_record(a, 2 + FW, &idx, &off)) {
+                                for (uint64_t k = lane; k < 2 + FW; k += kWave)
+                                    a.pool[off + k] = k == 0 ? (uint64_t)si : k == 1 ? base + i : Fi[k - 2];
+                                publish_record(a, idx, off);
+                                if (lane < W) frames[i * FW + 2 * W + lane] = 0;
+                            }
+                            break;
+                        }
+                    }
+                }
                 const bool p_any = ballot(pw != 0) != 0;
                 uint64_t xx_or = 0;
                 for (uint64_t i = lane; i < WX; i += kWave) xx_or |= XX[i];
@@ -362,12 +544,12 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                     for (uint64_t i = lane; i < WX; i += kWave) cx += __popcll(XX[i]);
                     np = wave_sum(__popcll(pw));
                     nPX = np + wave_sum(__popcll(xpw)) + wave_sum(cx);
-                    if (lane == 0 && own) my_bytes += 5 * (depth + 1 + nPX) * (depth ? 2 : 1);
+                    if (lane == 0 && own) my_bytes += 5 * (base + depth + 1 + nPX) * (base + depth ? 2 : 1);
                 }
                 if (!p_any) {
-                    if (!x_any && lane == 0) {  // maximal clique R, |R| = depth + 1
+                    if (!x_any && lane == 0) {  // maximal clique R, |R| = base + depth + 1
                         my_cliques++;
-                        const uint64_t sz = depth + 1;
+                        const uint64_t sz = base + depth + 1;
                         if (sz < 64) atomicAdd(&lhist[sz], 1ull);
                         else atomicAdd(&a.hist[sz < a.hist_len ? sz : a.hist_len - 1], 1ull);
                     }
@@ -441,7 +623,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                     pw &= ~m;
                     xpw |= m;
                 }
-                if (depth == 0 && (child++ % parts) != part) continue;  // another part's child
+                if (base + depth == 0 && (child++ % parts) != part) continue;  // another part's child
                 if (lane < W) {
                     F[lane] = pw;
                     F[W + lane] = xpw;
@@ -474,7 +656,28 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         }
         my_nodes += root_nodes;
         if (a.per_root && lane == 0 && root_nodes) atomicAdd(&a.per_root[si], (unsigned long long)root_nodes);
+    };
+
+    uint64_t roots_done = 0;
+    while (true) {
+        const uint64_t t = fetch_task(&a.ctr[CTR_NEXT]) + a.first;
+        if (t >= a.ntasks) break;
+        process(uniform(a.tasks ? a.tasks[t] : (uint64_t)t), nullptr);
+        roots_done++;
         t_end = wall_clock64();
+    }
+    if (a.donate_every) {
+        dq_add(a, DQ_IDLE, 1);
+        dq_add(a, DQ_DONE, roots_done);
+        while (true) {
+            const uint64_t off = claim_record(a, a.ntasks - a.first);
+            if (off == ~0ull) break;
+            const uint64_t* rec = a.pool + off;
+            process(uniform(rec[0]), rec);
+            t_end = wall_clock64();
+            dq_add(a, DQ_IDLE, 1);
+            dq_add(a, DQ_BUSY, ~0ull);
+        }
     }
     if (a.wave_span && lane == 0) {
         a.wave_span[2 * gwave] = t_start;
@@ -505,7 +708,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     const bool account = flags & G2_RUN_ACCOUNT;
     const uint32_t hl = hist ? hist_len : 2;
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
-    const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * c->tune.waves_per_cu / bw);
+    const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_MCE) / bw);
     const uint64_t nwaves = nblocks * bw;
     const uint64_t sw = c->tune.wave_scratch_words;
     int rc;
@@ -536,6 +739,21 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         ntasks = h.size();
     }
     if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks, 1) * 8, "deferred roots"))) return rc;
+    // donation queue: counters, record table and a word pool (a record is a frame plus two words)
+    const uint32_t every = account ? 0u : (c->tune.mce_donate_every == 0xffffffffu ? 0u : c->tune.mce_donate_every);
+    const uint64_t max_records = 1ull << 16, pool_words = 1ull << 23;
+    const uint64_t dq_words = DQ_N * kDqStride;
+    if ((rc = ensure(c, c->donate, dq_words * 8 + max_records * 12 + pool_words * 8, "donation queue"))) return rc;
+    auto* dq = (unsigned long long*)c->donate.ptr;
+    auto* rec_off = dq + dq_words;
+    auto* rec_ready = (unsigned int*)(rec_off + max_records);
+    auto* pool = (uint64_t*)(rec_ready + max_records);
+    auto reset_queue = [&](uint64_t waves) -> hipError_t {
+        (void)waves;
+        hipError_t err = hipMemsetAsync(dq, 0, dq_words * 8, c->stream);
+        if (err == hipSuccess) err = hipMemsetAsync(rec_ready, 0, max_records * 4, c->stream);
+        return err;
+    };
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
     if ((e = hipMemsetAsync(ctr, 0, CTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
@@ -558,6 +776,13 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     a.per_root = per_root_nodes ? (unsigned long long*)c->per_root.ptr : nullptr;
     a.wave_span = (unsigned long long*)c->spans.ptr;
     a.small_max = std::min<uint32_t>(c->tune.mce_small_max, kWave);
+    a.dq = dq;
+    a.pool = pool;
+    a.pool_words = pool_words;
+    a.rec_off = rec_off;
+    a.rec_ready = rec_ready;
+    a.max_records = max_records;
+    a.donate_every = every;
 
     auto launch = [&](uint64_t blocks) {
         if (account) hipLaunchKernelGGL(mce_kernel<true>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);
@@ -566,6 +791,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     };
     float k1 = 0, k2 = 0, tot = 0;
     uint32_t launches = 1;
+    if ((e = reset_queue(nwaves))) return hip_err(c, e, "donation queue reset");
     if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
     if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
     if ((e = launch(nblocks))) return hip_err(c, e, "mce launch");
@@ -587,6 +813,10 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         }
         if (hi > lo) idle = 1.0 - busy / ((double)nwaves * (double)(hi - lo));
     }
+    unsigned long long h_made = 0;
+    if ((e = hipMemcpy(&h_made, dq + DQ_MADE * kDqStride, 8, hipMemcpyDeviceToHost)))
+        return hip_err(c, e, "D2H donation counters");
+    uint64_t donations = h_made;
     const uint64_t nbig = h_ctr[CTR_NBIG];
     if (nbig) {
         // large-root pass: arenas sized for the largest deferred root
@@ -603,6 +833,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         a.big_list = nullptr;
         a.wave_span = nullptr;
         if ((e = hipMemsetAsync(ctr + CTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
+        if ((e = reset_queue(bwaves))) return hip_err(c, e, "donation queue reset");
         if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
         if ((e = launch(bwaves / bw))) return hip_err(c, e, "mce large-root launch");
         if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
@@ -615,7 +846,12 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         (e = hipMemcpyAsync(per_root_nodes, c->per_root.ptr, c->ns * 8, hipMemcpyDeviceToHost, c->stream)))
         return hip_err(c, e, "D2H per-root");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
-    if (nbig) (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
+    if (nbig) {
+        (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
+        if ((e = hipMemcpy(&h_made, dq + DQ_MADE * kDqStride, 8, hipMemcpyDeviceToHost)))
+            return hip_err(c, e, "D2H donation counters");
+        donations += h_made;
+    }
     (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[3]);
     if (h_ctr[CTR_ERR])
         return set_err(c, G2_EBUDGET, "a root needs more than 4096 later neighbours or did not fit the large-root arena");
@@ -629,6 +865,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         st->big_roots = nbig;
         st->max_scratch_words = h_ctr[CTR_MAXNEED];
         st->wave_idle_frac = idle;
+        st->donations = donations;
     }
     return G2_OK;
 }

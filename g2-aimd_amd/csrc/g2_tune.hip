@@ -39,7 +39,7 @@ extern "C" int g2_gm_autotune(g2_ctx* c, const g2_plan* plans, uint32_t nq, uint
     g2_batch cur{};
     cur.window = 256;
     cur.roots_per_task = 1;
-    cur.waves_per_cu = c->tune.waves_per_cu;
+    cur.waves_per_cu = waves_per_cu(c, WK_GM);
     cur.node_budget = budget;
     std::vector<uint64_t> ref(sample.size()), got(sample.size());
     std::vector<uint32_t> over(sample.size());
@@ -69,7 +69,7 @@ extern "C" int g2_gm_autotune(g2_ctx* c, const g2_plan* plans, uint32_t nq, uint
     if ((rc = trial(cur, &cur_ms))) return rc;
     const uint32_t windows[] = {0, 64, 256, 1024, 4096};
     const uint32_t rpts[] = {1, 2, 8, 32};
-    const uint32_t wpcs[] = {4, 8, 12, 16};
+    const uint32_t wpcs[] = {8, 16, 24, 32};
     for (uint32_t w : windows) {
         if (w == cur.window) continue;
         g2_batch b = cur;

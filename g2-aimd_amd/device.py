@@ -36,13 +36,15 @@ G2_MLP_NPARAM = 4801
 class Tuning(C.Structure):
     _fields_ = [("waves_per_cu", C.c_uint32), ("block_waves", C.c_uint32),
                 ("wave_scratch_words", C.c_uint64), ("big_waves", C.c_uint32),
-                ("gm_chunk", C.c_uint32), ("mce_small_max", C.c_uint32)]
+                ("gm_chunk", C.c_uint32), ("mce_small_max", C.c_uint32),
+                ("mce_donate_every", C.c_uint32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("launches", C.c_uint32), ("total_ms", C.c_double),
                 ("nodes", C.c_uint64), ("alg_bytes", C.c_uint64), ("big_roots", C.c_uint64),
-                ("max_scratch_words", C.c_uint64), ("wave_idle_frac", C.c_double)]
+                ("max_scratch_words", C.c_uint64), ("wave_idle_frac", C.c_double),
+                ("donations", C.c_uint64)]
 
     def to_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -127,6 +127,7 @@ def experiment(g2, dev, scale: int, train_seeds, test_seed: int, epochs: int = 3
     log(f"trained on {len(X)} roots: last-epoch mse {loss:.4f}")
 
     g, core = prepare_graph(g2, scale, test_seed)
+    out = {"scale": scale, "train_roots": int(len(X)), "test_roots": int(g.nv), "train_mse": loss}
     with dev.Executor(device=0, **tuning) as ex:
         ex.transfer(g)
         x = root_features(ex, np.arange(g.nv, dtype=np.uint32), core, g.nv)
@@ -134,31 +135,34 @@ def experiment(g2, dev, scale: int, train_seeds, test_seed: int, epochs: int = 3
         plain = ex.run_mce(per_root=True)
         truth = np.log1p(plain["per_root_nodes"].astype(np.float64))
         info = dev.device_info(0)
-        waves = info["num_cus"] * max(1, tuning.get("waves_per_cu", 0) or 8)
+        waves = info["num_cus"] * max(1, tuning.get("waves_per_cu", 0) or 16)
         later = ex.features()[:, 4]
         order, split = learned_schedule(pred, later, waves)
-        learned = ex.run_mce(order=order, split=split)
-        if learned["total"] != plain["total"] or not np.array_equal(learned["hist"], plain["hist"]):
-            raise RuntimeError("learned schedule changed the result")
-        tp, tl = [], []
-        for _ in range(reps):
-            tp.append(ex.run_mce()["stats"])
-            tl.append(ex.run_mce(order=order, split=split)["stats"])
     mlp.close()
     nodes = plain["per_root_nodes"].astype(np.float64)
-    ident = np.arange(g.nv)
-    ms_p = float(np.median([s["kernel_ms"] for s in tp]))
-    ms_l = float(np.median([s["kernel_ms"] for s in tl]))
-    return {
-        "scale": scale, "train_roots": int(len(X)), "test_roots": int(g.nv),
-        "max_cliques": int(plain["total"]), "train_mse": loss,
+    out.update({
+        "max_cliques": int(plain["total"]),
         "rmse_log_nodes": float(np.sqrt(np.mean((pred - truth) ** 2))),
         "spearman": spearman(pred, truth),
-        "plain_ms": ms_p, "learned_ms": ms_l, "speedup": ms_p / ms_l,
-        "plain_idle_frac": float(np.median([s["wave_idle_frac"] for s in tp])),
-        "learned_idle_frac": float(np.median([s["wave_idle_frac"] for s in tl])),
         "split_roots": int((split > 1).sum()), "split_parts": int(split.sum()),
-        "sim_makespan_plain": simulated_makespan(nodes, ident, None, waves),
+        "sim_makespan_plain": simulated_makespan(nodes, np.arange(g.nv), None, waves),
         "sim_makespan_learned": simulated_makespan(nodes, order, split, waves),
         "sim_makespan_bound": max(nodes.sum() / waves, 1.0),
-    }
+    })
+    # wall time of the plain and learned schedules, with and without work donation
+    for dname, extra in (("donate", {}), ("nodonate", {"mce_donate_every": 0xFFFFFFFF})):
+        with dev.Executor(device=0, **{**tuning, **extra}) as ex:
+            ex.transfer(g)
+            for sname, kw in (("plain", {}), ("learned", {"order": order, "split": split})):
+                r = ex.run_mce(**kw)
+                if r["total"] != plain["total"] or not np.array_equal(r["hist"], plain["hist"]):
+                    raise RuntimeError("a schedule changed the result")
+                st = [ex.run_mce(**kw)["stats"] for _ in range(reps)]
+                key = f"{sname}_{dname}"
+                out[key + "_ms"] = float(np.median([s["kernel_ms"] for s in st]))
+                out[key + "_idle_frac"] = float(np.median([s["wave_idle_frac"] for s in st]))
+                out[key + "_donations"] = int(st[-1]["donations"])
+                log(f"{key}: {out[key + '_ms']:.2f} ms, idle {out[key + '_idle_frac']:.3f}")
+    out["speedup_learned_vs_plain"] = out["plain_donate_ms"] / out["learned_donate_ms"]
+    out["speedup_learned_vs_plain_nodonate"] = out["plain_nodonate_ms"] / out["learned_nodonate_ms"]
+    return out

@@ -44,13 +44,16 @@ typedef struct g2_ctx g2_ctx;
 
 /* launch and scratch parameters (0 = library default) */
 typedef struct g2_tuning {
-    uint32_t waves_per_cu;        /* resident search waves per CU of the persistent grid */
+    uint32_t waves_per_cu;        /* resident search waves per CU of the persistent grids
+                                     (0 = each kernel's default: 16, local-bitmap matching 32) */
     uint32_t block_waves;         /* waves per workgroup */
     uint64_t wave_scratch_words;  /* per-wave u64 words for MCE local bitsets (small-root pass) */
     uint32_t big_waves;           /* waves of the large-root pass */
     uint32_t gm_chunk;            /* level-1 candidates per matching task (0 = whole root) */
     uint32_t mce_small_max;       /* MCE roots of at most this degree run in registers
                                      (0 = 64, the most; 1 = practically off) */
+    uint32_t mce_donate_every;    /* MCE: search nodes between a wave's checks for idle waves,
+                                     which then take its open frames (0 = 32; 0xffffffff = never) */
 } g2_tuning;
 
 /* measurements of the last run */
@@ -64,6 +67,7 @@ typedef struct g2_stats {
     uint64_t max_scratch_words;   /* largest per-root scratch need seen */
     double   wave_idle_frac;      /* MCE: 1 - sum of per-wave busy spans / (waves x makespan),
                                      from the GPU real-time clock (tail idleness) */
+    uint64_t donations;           /* MCE: search frames handed from busy to idle waves */
 } g2_stats;
 
 /* root schedule for g2_mce_run (NULL = the given source order, one wave per root) */

@@ -98,7 +98,8 @@ def test_large_root_pass_and_schedule(g2, orc):
     print("case: large-root pass", flush=True)
     g = G.gnp(g2, 400, 0.08, 21)
     ref = orc.mce(g, per_root=True)
-    with g2.Executor(device=0, wave_scratch_words=64, big_waves=64) as ex:
+    # mce_small_max=1 keeps these degree-~32 roots off the register path
+    with g2.Executor(device=0, wave_scratch_words=64, big_waves=64, mce_small_max=1) as ex:
         ex.transfer(g)
         got = ex.run_mce(per_root=True)
         assert got["stats"]["big_roots"] > 0
@@ -139,3 +140,31 @@ def test_rmat_mce(g2, orc, gpu_exec):
     g = g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000005).preprocess()
     g, _, _, _ = g.kcore_recode()
     check(gpu_exec, orc, g, account=True)
+
+
+@pytest.mark.parametrize("every", [1, 7, 0xFFFFFFFF])
+def test_work_donation(g2, orc, every):
+    """Busy waves hand open frames to idle waves (every `every` nodes they look
+    for idle waves; 0xFFFFFFFF = never): the tree, every count and per-root
+    node total are unchanged.  R-MAT roots are skewed, so idle waves appear."""
+    print("case: work donation every", every, flush=True)
+    g = g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000005).preprocess()
+    g, _, _, _ = g.kcore_recode()
+    ref = orc.mce(g, per_root=True)
+    for small in (64, 1):  # register path and arena path
+        with g2.Executor(device=0, mce_donate_every=every, mce_small_max=small) as ex:
+            ex.transfer(g)
+            got = ex.run_mce(per_root=True)
+            assert got["total"] == ref["total"]
+            assert np.array_equal(got["hist"], ref["hist"])
+            assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+            if every == 0xFFFFFFFF:
+                assert got["stats"]["donations"] == 0
+            else:
+                assert got["stats"]["donations"] > 0
+            # split roots and donation together
+            split = np.full(g.nv, 3, np.uint32)
+            got = ex.run_mce(per_root=True, split=split)
+            assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+            assert np.array_equal(got["hist"], ref["hist"])
+
