@@ -109,6 +109,8 @@ def host_lib() -> C.CDLL:
     vp = C.c_void_p
     pp = C.POINTER(C.c_void_p)
     L.g2h_last_error.restype = C.c_char_p
+    L.g2h_set_progress.argtypes = [C.c_int]
+    L.g2h_set_progress.restype = None
     L.g2h_graph_read_bin.argtypes = [C.c_char_p, pp]
     L.g2h_graph_write_bin.argtypes = [vp, C.c_char_p]
     L.g2h_graph_from_csr.argtypes = [u64p, C.c_uint64, u32p, C.c_uint64, pp]
@@ -268,6 +270,11 @@ def gen_rmat(scale, edge_factor, a, b, c, seed) -> Graph:
     h = C.c_void_p()
     _hcheck(host_lib().g2h_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)), "gen_rmat")
     return Graph._from_handle(h)
+
+
+def set_progress(on: bool = True) -> None:
+    """progress lines on stderr from long host steps"""
+    host_lib().g2h_set_progress(1 if on else 0)
 
 
 def gen_social(n, gamma, w_min, w_max, backbone_edges, ncomm, smin, smax, comm_alpha,

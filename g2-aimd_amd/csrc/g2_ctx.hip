@@ -101,7 +101,7 @@ int g2_create(g2_ctx** out, int device, uint64_t budget, const g2_tuning* t) {
     if (!c->tune.wave_scratch_words) c->tune.wave_scratch_words = 1ull << 16;
     if (!c->tune.big_waves) c->tune.big_waves = 256;
     if (!c->tune.mce_small_max) c->tune.mce_small_max = 64;
-    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 32;
+    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return G2_EHIP; }
     for (auto& ev : c->ev)
         if (hipEventCreate(&ev) != hipSuccess) { g2_destroy(c); return G2_EHIP; }

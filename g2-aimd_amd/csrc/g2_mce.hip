@@ -86,9 +86,6 @@ __device__ inline void arena_sync() { wave_sync(); }
 
 __device__ inline uint64_t degree_of(const uint64_t* __restrict__ rp, uint32_t u) { return rp[u + 1] - rp[u]; }
 
-__device__ inline unsigned long long ld_agent(unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Donation queue.  A record is [source index, absolute depth, P (W words),
 // XP (W), Q (W), XX (WX)]: a frame whose untried children Q the taker walks
@@ -102,16 +99,31 @@ __device__ inline unsigned long long ld_agent(unsigned long long* p) {
 // idle wave exits once every root task is done, no wave holds a record and
 // every reserved record has been taken.  A wave that published records
 // claims any still unclaimed before it can exit, so none is left behind.
-// Only these per-wave transitions and the (rare) donations use acquire /
-// release at device scope; the per-root fetch stays a relaxed atomic.
+//
+// Coherence without cache maintenance.  Device-scope acquire/release would
+// write back or invalidate an XCD's whole L2, which idle waves polling the
+// queue would do continuously.  Instead every queue word and every record
+// word is accessed with device-scope atomics (coherent across XCDs without
+// touching the rest of the L2), and order between a wave's own accesses
+// comes from waiting for its stores to complete (s_waitcnt) before the next
+// one is issued, and from issuing a load only after the value it depends on
+// has arrived.
+__device__ inline unsigned long long ld_dev(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until this wave's memory operations have completed
+__device__ inline void drain() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
 
 // true when idle waves outnumber the records waiting for them
 __device__ inline bool hungry(const MceArgs& a) {
     __builtin_amdgcn_wave_barrier();
     uint32_t h = 0;
     if (lane_id() == 0) {
-        const uint64_t idle = ld_agent(&a.dq[DQ_IDLE * kDqStride]);
-        const uint64_t res = ld_agent(&a.dq[DQ_RES * kDqStride]), taken = ld_agent(&a.dq[DQ_TAKEN * kDqStride]);
+        const uint64_t idle = ld_dev(&a.dq[DQ_IDLE * kDqStride]);
+        const uint64_t res = ld_dev(&a.dq[DQ_RES * kDqStride]), taken = ld_dev(&a.dq[DQ_TAKEN * kDqStride]);
         h = idle > res - taken;
     }
     return uniform(h) != 0;
@@ -124,7 +136,7 @@ __device__ inline bool reserve_record(const MceArgs& a, uint64_t words, uint64_t
     if (lane_id() == 0) {
         o = atomicAdd(&a.dq[DQ_TOP * kDqStride], (unsigned long long)words);
         if (o + words <= a.pool_words) {
-            unsigned long long c = ld_agent(&a.dq[DQ_RES * kDqStride]);
+            unsigned long long c = ld_dev(&a.dq[DQ_RES * kDqStride]);
             while (c < a.max_records) {
                 const unsigned long long prev = atomicCAS(&a.dq[DQ_RES * kDqStride], c, c + 1);
                 if (prev == c) { i = c; break; }
@@ -137,22 +149,26 @@ __device__ inline bool reserve_record(const MceArgs& a, uint64_t words, uint64_t
     return *idx != ~0ull;
 }
 
-// the record words are written by the whole wave: make them visible, then flag
+// the record words were stored (st_dev) by the whole wave: once they have
+// completed, flag the record
 __device__ inline void publish_record(const MceArgs& a, uint64_t idx, uint64_t off) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    drain();
     __builtin_amdgcn_wave_barrier();
     if (lane_id() == 0) {
-        __hip_atomic_store(&a.rec_off[idx], (unsigned long long)off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        st_dev(&a.rec_off[idx], off);
+        drain();
+        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&a.dq[DQ_MADE * kDqStride], 1ull);
     }
+    drain();
 }
 
-// add (wrapping) to a queue counter with release order: after the wave's publishes
+// add (wrapping) to a queue counter after this wave's earlier stores completed
 __device__ inline void dq_add(const MceArgs& a, int which, unsigned long long v) {
+    drain();
     __builtin_amdgcn_wave_barrier();
-    if (lane_id() == 0)
-        __hip_atomic_fetch_add(&a.dq[which * kDqStride], v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane_id() == 0) atomicAdd(&a.dq[which * kDqStride], v);
+    drain();
 }
 
 // Next record for an idle wave (counted in DQ_IDLE): its pool offset, or ~0
@@ -162,30 +178,26 @@ __device__ inline uint64_t claim_record(const MceArgs& a, uint64_t total_tasks) 
         __builtin_amdgcn_wave_barrier();
         unsigned long long got = ~0ull - 1;  // nothing yet
         if (lane_id() == 0) {
-            const unsigned long long taken = ld_agent(&a.dq[DQ_TAKEN * kDqStride]);
-            const unsigned long long res = ld_agent(&a.dq[DQ_RES * kDqStride]);
+            const unsigned long long taken = ld_dev(&a.dq[DQ_TAKEN * kDqStride]);
+            const unsigned long long res = ld_dev(&a.dq[DQ_RES * kDqStride]);
             if (taken < res) {
                 if (atomicCAS(&a.dq[DQ_TAKEN * kDqStride], taken, taken + 1) == taken) {
                     atomicAdd(&a.dq[DQ_BUSY * kDqStride], 1ull);
                     atomicAdd(&a.dq[DQ_IDLE * kDqStride], ~0ull);
                     // the publisher is inside a task, so the flag comes
-                    while (__hip_atomic_load(&a.rec_ready[taken], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
-                        __builtin_amdgcn_s_sleep(1);
-                    got = __hip_atomic_load(&a.rec_off[taken], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    while (__hip_atomic_load(&a.rec_ready[taken], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                        __builtin_amdgcn_s_sleep(2);
+                    got = ld_dev(&a.rec_off[taken]);
                 }
-            } else if (__hip_atomic_load(&a.dq[DQ_DONE * kDqStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == total_tasks &&
-                       __hip_atomic_load(&a.dq[DQ_BUSY * kDqStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-                       ld_agent(&a.dq[DQ_TAKEN * kDqStride]) == ld_agent(&a.dq[DQ_RES * kDqStride])) {
+            } else if (ld_dev(&a.dq[DQ_DONE * kDqStride]) == total_tasks && ld_dev(&a.dq[DQ_BUSY * kDqStride]) == 0 &&
+                       ld_dev(&a.dq[DQ_TAKEN * kDqStride]) == ld_dev(&a.dq[DQ_RES * kDqStride])) {
                 atomicAdd(&a.dq[DQ_IDLE * kDqStride], ~0ull);
                 got = ~0ull;
             }
         }
         got = uniform((uint64_t)got);
-        if (got != ~0ull - 1) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            return got;
-        }
-        __builtin_amdgcn_s_sleep(32);
+        if (got != ~0ull - 1) return got;
+        __builtin_amdgcn_s_sleep(127);
     }
 }
 
@@ -287,11 +299,13 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
     uint32_t base = 0;
     bool fresh = true;
     if (rec) {
-        base = (uint32_t)uniform(rec[1]);
-        P = uniform(rec[2]);
-        XP = uniform(rec[3]);
-        Q = uniform(rec[4]);
-        XX = uniform(rec[5]);
+        const unsigned long long* r = (const unsigned long long*)rec;
+        const uint64_t w = lane < 6 ? ld_dev(r + lane) : 0;
+        base = (uint32_t)read_lane(w, 1);
+        P = read_lane(w, 2);
+        XP = read_lane(w, 3);
+        Q = read_lane(w, 4);
+        XX = read_lane(w, 5);
         fresh = false;
     }
     uint64_t fP = 0, fXP = 0, fXX = 0, fQ = 0;  // frame of depth == lane
@@ -301,7 +315,9 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
         bool descend = false;
         if (fresh) {
             if (base + depth > 0 || part == 0) nodes++;
-            if (a.donate_every && ++since >= a.donate_every) {
+            // a task that has walked 16 x donate_every nodes is heavy: from then on,
+            // every donate_every nodes, look for idle waves
+            if (a.donate_every && ++since >= a.donate_every && nodes >= 16ull * a.donate_every) {
                 since = 0;
                 // hand the shallowest frame with untried children to an idle wave
                 // (not the root frame of a split root: its children are dealt by part)
@@ -315,7 +331,7 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
                                        fxx = read_lane(fXX, i);
                         const uint64_t w = lane == 0 ? (uint64_t)si : lane == 1 ? (uint64_t)(base + i)
                                          : lane == 2 ? fp : lane == 3 ? fxp : lane == 4 ? fq : fxx;
-                        if (lane < 6) a.pool[off + lane] = w;
+                        if (lane < 6) st_dev((unsigned long long*)&a.pool[off + lane], w);
                         publish_record(a, idx, off);
                         fQ = lane == i ? 0 : fQ;
                     }
@@ -491,8 +507,8 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         uint64_t base = 0;   // depth of the start node (a donated frame's depth)
         bool fresh = true;
         if (rec) {
-            base = uniform(rec[1]);
-            for (uint64_t i = lane; i < FW; i += kWave) frames[i] = rec[2 + i];
+            base = uniform((uint64_t)ld_dev((const unsigned long long*)rec + 1));
+            for (uint64_t i = lane; i < FW; i += kWave) frames[i] = ld_dev((const unsigned long long*)rec + 2 + i);
             arena_sync();
             if (lane < W) {
                 pw = frames[lane];
@@ -515,7 +531,8 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
             const bool own = base + depth > 0 || part == 0;
             if (fresh) {
                 if (own) root_nodes++;
-                if (!ACCOUNT && a.donate_every && ++since >= a.donate_every) {
+                if (!ACCOUNT && a.donate_every && ++since >= a.donate_every &&
+                    root_nodes >= 16ull * a.donate_every) {
                     since = 0;
                     if (hungry(a)) {
                         // hand the shallowest ancestor frame with untried children to an
@@ -526,7 +543,8 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                             uint64_t idx, off;
                             if (reserve_record(a, 2 + FW, &idx, &off)) {
                                 for (uint64_t k = lane; k < 2 + FW; k += kWave)
-                                    a.pool[off + k] = k == 0 ? (uint64_t)si : k == 1 ? base + i : Fi[k - 2];
+                                    st_dev((unsigned long long*)&a.pool[off + k],
+                                           k == 0 ? (uint64_t)si : k == 1 ? base + i : Fi[k - 2]);
                                 publish_record(a, idx, off);
                                 if (lane < W) frames[i * FW + 2 * W + lane] = 0;
                             }
@@ -673,7 +691,7 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
             const uint64_t off = claim_record(a, a.ntasks - a.first);
             if (off == ~0ull) break;
             const uint64_t* rec = a.pool + off;
-            process(uniform(rec[0]), rec);
+            process(uniform((uint64_t)ld_dev((const unsigned long long*)rec)), rec);
             t_end = wall_clock64();
             dq_add(a, DQ_IDLE, 1);
             dq_add(a, DQ_BUSY, ~0ull);
@@ -703,6 +721,191 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         if (lhist[i]) atomicAdd(&a.hist[i < a.hist_len ? i : a.hist_len - 1], lhist[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Batched small roots: 64/G roots per wave, one per group of G lanes (G = 16
+// or 32).  Every root here has d <= G, so a node is four G-bit words held by
+// all lanes of its group, position k of C maps to lane k of the group, and
+// the group's DFS stack sits one frame per lane, as in the register path.
+// Group-level exchanges use lane permutes (ds_bpermute) and xor shuffles
+// below G, so groups never read each other's lanes.  Same tree, pivot rule
+// and child order as the other paths; these roots are never split or donated.
+// ---------------------------------------------------------------------------
+template <int G>
+__device__ inline uint32_t group_read(uint32_t x, uint32_t gbase, uint32_t idx) {
+    return (uint32_t)__shfl((int)x, (int)(gbase + idx), kWave);
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
+    static_assert(G == 16 || G == 32, "group width");
+    constexpr uint32_t K = kWave / G;
+    __shared__ unsigned long long lhist[64];
+    __shared__ SmallLds small_lds[kMaxBlockWaves];
+    for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x) lhist[i] = 0;
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint32_t gl = lane % G, gbase = lane - gl;
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    SmallLds& L = small_lds[threadIdx.x / kWave];
+    uint64_t my_nodes = 0, my_cliques = 0, hcount = 0, hbig = 0;
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_end = t_start;
+
+    while (true) {
+        __builtin_amdgcn_wave_barrier();
+        unsigned long long t0 = 0;
+        if (lane == 0) t0 = atomicAdd(&a.ctr[CTR_NEXT], (unsigned long long)K);
+        t0 = uniform((uint64_t)t0);
+        if (t0 >= a.ntasks) break;
+        // ---- this group's root ----
+        const uint64_t t = t0 + lane / G;
+        const bool valid = t < a.ntasks;
+        const uint32_t si = valid ? (uint32_t)a.tasks[t] : 0u;
+        const uint32_t v = valid ? a.src[si] : 0u;
+        const uint64_t lo = valid ? a.rp[v] : 0;
+        const uint32_t d = valid ? (uint32_t)(a.rp[v + 1] - lo) : 0u;
+        if (gl < d) L.C[lane] = a.cols[lo + gl];
+        L.adjP[lane] = 0;
+        L.adjX[lane] = 0;
+        wave_sync();
+        const uint32_t gmask = G == 32 ? 0xffffffffu : 0xffffu;
+        const uint32_t smask = (uint32_t)(ballot(gl < d && L.C[lane] < v) >> gbase) & gmask;
+        const uint32_t s = __popc(smask), p0 = d - s;
+        uint32_t len = 0;
+        if (gl < p0) {
+            const uint32_t p = L.C[gbase + s + gl];
+            uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
+            if (e1 - e0 > kWave) {
+                e0 = lower_bound(a.cols, e0, e1, L.C[gbase]);
+                e1 = lower_bound(a.cols, e0, e1, L.C[gbase + d - 1] + 1);
+            }
+            L.start[lane] = e0;
+            len = (uint32_t)(e1 - e0);
+        }
+        uint32_t incl = len;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, G);
+            if (gl >= (uint32_t)o) incl += y;
+        }
+        L.pref[lane] = incl - len;
+        const uint32_t T = group_read<G>(incl, gbase, G - 1);
+        wave_sync();
+        for (uint32_t f = gl; f < T; f += G) {
+            uint32_t lo2 = 0, hi2 = p0;
+            while (hi2 - lo2 > 1) {
+                const uint32_t mid = (lo2 + hi2) >> 1;
+                if (L.pref[gbase + mid] <= f) lo2 = mid; else hi2 = mid;
+            }
+            const uint32_t j = lo2;
+            const uint32_t w = a.cols[L.start[gbase + j] + (f - L.pref[gbase + j])];
+            const uint32_t k = lds_lower_bound(L.C + gbase, 0, d, w);
+            if (k < d && L.C[gbase + k] == w) {
+                if (k < s) {
+                    atomicOr((unsigned long long*)&L.adjX[gbase + j], 1ull << k);
+                    atomicOr((unsigned long long*)&L.adjP[gbase + k], 1ull << j);
+                } else {
+                    atomicOr((unsigned long long*)&L.adjP[gbase + s + j], 1ull << (k - s));
+                }
+            }
+        }
+        wave_sync();
+        const uint32_t rowP = (uint32_t)L.adjP[lane];
+        const uint32_t rowX = (uint32_t)L.adjX[lane];
+
+        // ---- per-group depth-first walks, stepped together ----
+        uint32_t P = p0 >= 32 ? 0xffffffffu : (1u << p0) - 1, XP = 0;
+        uint32_t XX = s >= 32 ? 0xffffffffu : (1u << s) - 1, Q = 0;
+        uint32_t fP = 0, fXP = 0, fXX = 0, fQ = 0;
+        uint32_t depth = 0;
+        uint64_t nodes = 0;
+        bool fresh = true;
+        bool done = d == 0;
+        if (!done && p0 == 0) {  // empty P, non-empty X: one node, no clique
+            nodes = 1;
+            done = true;
+        }
+        while (ballot(!done)) {
+            if (!done) {
+                bool descend = false;
+                if (fresh) {
+                    nodes++;
+                    if (P == 0) {
+                        if (XP == 0 && XX == 0) {
+                            my_cliques += gl == 0;
+                            const uint32_t sz = depth + 1;
+                            if (sz < G) hcount += (gl == sz);
+                            else hbig += (gl == sz - G);
+                        }
+                    } else {
+                        const uint32_t PX = P | XP;
+                        const bool cand = gl < s ? ((XX >> gl) & 1) : (gl < d && ((PX >> (gl - s)) & 1));
+                        uint32_t key = cand ? ((uint32_t)__popc(rowP & P) << 8) | gl : 0u;
+#pragma unroll
+                        for (int m = G / 2; m >= 1; m >>= 1) {
+                            const uint32_t y = (uint32_t)__shfl_xor((int)key, m, kWave);
+                            key = y > key ? y : key;
+                        }
+                        Q = P & ~group_read<G>(rowP, gbase, key & 0xffu);
+                    }
+                    fresh = false;
+                }
+                if (Q != 0) {
+                    const uint32_t j = __builtin_ctz(Q);
+                    const uint32_t m = 1u << j;
+                    Q &= ~m;
+                    P &= ~m;
+                    XP |= m;
+                    const bool mine = gl == depth;
+                    fP = mine ? P : fP;
+                    fXP = mine ? XP : fXP;
+                    fXX = mine ? XX : fXX;
+                    fQ = mine ? Q : fQ;
+                    const uint32_t qrow = group_read<G>(rowP, gbase, s + j);
+                    P &= qrow;
+                    XP &= qrow;
+                    XX &= group_read<G>(rowX, gbase, j);
+                    Q = 0;
+                    depth++;
+                    fresh = true;
+                    descend = true;
+                }
+                if (!descend) {
+                    if (depth == 0) {
+                        done = true;
+                    } else {
+                        depth--;
+                        P = group_read<G>(fP, gbase, depth);
+                        XP = group_read<G>(fXP, gbase, depth);
+                        XX = group_read<G>(fXX, gbase, depth);
+                        Q = group_read<G>(fQ, gbase, depth);
+                    }
+                }
+            }
+        }
+        if (valid && gl == 0) {
+            my_nodes += nodes;
+            if (a.per_root && nodes) atomicAdd(&a.per_root[si], (unsigned long long)nodes);
+        }
+        t_end = wall_clock64();
+    }
+    if (a.wave_span && lane == 0) {
+        a.wave_span[2 * gwave] = t_start;
+        a.wave_span[2 * gwave + 1] = t_end;
+    }
+    my_nodes = wave_sum(my_nodes);
+    my_cliques = wave_sum(my_cliques);
+    if (hcount) atomicAdd(&lhist[gl], (unsigned long long)hcount);
+    if (hbig) atomicAdd(&a.hist[min<uint32_t>(G + gl, a.hist_len - 1)], (unsigned long long)hbig);
+    if (lane == 0) {
+        if (my_cliques) atomicAdd(&a.ctr[CTR_CLIQUES], (unsigned long long)my_cliques);
+        if (my_nodes) atomicAdd(&a.ctr[CTR_NODES], (unsigned long long)my_nodes);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x)
+        if (lhist[i]) atomicAdd(&a.hist[i < a.hist_len ? i : a.hist_len - 1], lhist[i]);
+}
+
 int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
             uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
@@ -720,19 +923,32 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     // task list: sources in schedule order, heavy ones dealt into parts
     const uint64_t* tasks = nullptr;
     uint64_t first = 0, ntasks = c->ns;
-    if (sched && (sched->order || sched->split || sched->end > sched->begin)) {
-        const uint64_t b = sched->end > sched->begin ? sched->begin : 0;
-        const uint64_t en = sched->end > sched->begin ? sched->end : c->ns;
+    // batched small roots: roots of degree <= G go to the lane-group kernel
+    const uint32_t rpw = c->tune.mce_roots_per_wave;
+    const uint32_t G = account ? 0u : rpw >= 4 ? 16u : rpw >= 2 ? 32u : 0u;
+    std::vector<uint64_t> hs;  // source indices for the lane-group kernel
+    if (G || (sched && (sched->order || sched->split || sched->end > sched->begin))) {
+        const bool ranged = sched && sched->end > sched->begin;
+        const uint64_t b = ranged ? sched->begin : 0;
+        const uint64_t en = ranged ? sched->end : c->ns;
         std::vector<uint64_t> h;
         h.reserve(en - b);
         for (uint64_t t = b; t < en; t++) {
-            const uint64_t si = sched->order ? sched->order[t] : t;
-            const uint64_t k = sched->split ? std::max<uint32_t>(1, std::min<uint32_t>(sched->split[si], 0xffffu)) : 1;
+            const uint64_t si = sched && sched->order ? sched->order[t] : t;
+            const uint64_t k = sched && sched->split ? std::max<uint32_t>(1, std::min<uint32_t>(sched->split[si], 0xffffu)) : 1;
+            if (G && k == 1 && c->h_src_deg[si] <= G) {
+                hs.push_back(si);
+                continue;
+            }
             for (uint64_t j = 0; j < k; j++) h.push_back(si | (j << 32) | (k << 48));
         }
-        if ((rc = ensure(c, c->order, std::max<uint64_t>(h.size(), 1) * 8, "task list"))) return rc;
+        if ((rc = ensure(c, c->order, std::max<uint64_t>(h.size() + hs.size(), 1) * 8, "task list"))) return rc;
         if (!h.empty()) {
             hipError_t e = hipMemcpy(c->order.ptr, h.data(), h.size() * 8, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_err(c, e, "H2D task list");
+        }
+        if (!hs.empty()) {
+            hipError_t e = hipMemcpy((uint64_t*)c->order.ptr + h.size(), hs.data(), hs.size() * 8, hipMemcpyHostToDevice);
             if (e != hipSuccess) return hip_err(c, e, "H2D task list");
         }
         tasks = (const uint64_t*)c->order.ptr;
@@ -839,6 +1055,31 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
         launches = 2;
     }
+    float k3 = 0;
+    bool k2_done = false;
+    if (!hs.empty()) {
+        if (nbig) {  // the large-root pass's events are reused below
+            if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
+            (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
+            k2_done = true;
+        }
+        // lane-group kernel over the small roots
+        a.tasks = tasks + ntasks;
+        a.first = 0;
+        a.ntasks = hs.size();
+        a.wave_span = nullptr;
+        a.donate_every = 0;
+        const uint64_t gblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * (c->tune.waves_per_cu ? c->tune.waves_per_cu : 32) / bw);
+        if ((e = hipMemsetAsync(ctr + CTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
+        if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
+        if (G == 16) hipLaunchKernelGGL(mce_group_kernel<16>, dim3(gblocks), dim3(bw * kWave), 0, c->stream, a);
+        else hipLaunchKernelGGL(mce_group_kernel<32>, dim3(gblocks), dim3(bw * kWave), 0, c->stream, a);
+        if ((e = hipGetLastError())) return hip_err(c, e, "mce lane-group launch");
+        if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
+        if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce lane-group sync");
+        (void)hipEventElapsedTime(&k3, c->ev[1], c->ev[2]);
+        launches++;
+    }
     if ((e = hipEventRecord(c->ev[3], c->stream))) return hip_err(c, e, "event");
     if ((e = hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
     if (hist && (e = hipMemcpyAsync(hist, c->hist.ptr, (uint64_t)hl * 8, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H hist");
@@ -847,7 +1088,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         return hip_err(c, e, "D2H per-root");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
     if (nbig) {
-        (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
+        if (!k2_done) (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
         if ((e = hipMemcpy(&h_made, dq + DQ_MADE * kDqStride, 8, hipMemcpyDeviceToHost)))
             return hip_err(c, e, "D2H donation counters");
         donations += h_made;
@@ -857,7 +1098,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         return set_err(c, G2_EBUDGET, "a root needs more than 4096 later neighbours or did not fit the large-root arena");
     if (total) *total = h_ctr[CTR_CLIQUES];
     if (st) {
-        st->kernel_ms = (double)k1 + (double)k2;
+        st->kernel_ms = (double)k1 + (double)k2 + (double)k3;
         st->launches = launches;
         st->total_ms = tot;
         st->nodes = h_ctr[CTR_NODES];

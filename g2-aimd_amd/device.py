@@ -37,7 +37,7 @@ class Tuning(C.Structure):
     _fields_ = [("waves_per_cu", C.c_uint32), ("block_waves", C.c_uint32),
                 ("wave_scratch_words", C.c_uint64), ("big_waves", C.c_uint32),
                 ("gm_chunk", C.c_uint32), ("mce_small_max", C.c_uint32),
-                ("mce_donate_every", C.c_uint32)]
+                ("mce_donate_every", C.c_uint32), ("mce_roots_per_wave", C.c_uint32)]
 
 
 class Stats(C.Structure):

@@ -17,6 +17,8 @@ namespace g2h {
 
 // per-thread last error
 void set_error(const std::string& msg);
+// progress line on stderr for long host steps (off unless g2h_set_progress(1))
+void progress(const char* fmt, ...);
 
 // splitmix64-seeded xoshiro256** (Blackman & Vigna, "Scrambled linear
 // pseudorandom number generators", 2018)

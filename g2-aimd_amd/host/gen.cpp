@@ -197,6 +197,7 @@ int g2h_gen_social(uint64_t n, double gamma, double w_min, double w_max, uint64_
     const uint64_t pairs_max = coff[ncomm];
     std::vector<uint32_t> a(mb + pairs_max), b(mb + pairs_max);
     draw_pairs(al, mb, seed, a, b, 0);
+    g2h::progress("social: %llu backbone pairs drawn", (unsigned long long)mb);
     std::vector<uint8_t> keep(pairs_max, 0);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t i = 0; i < (int64_t)ncomm; i++) {
@@ -224,8 +225,10 @@ int g2h_gen_social(uint64_t n, double gamma, double w_min, double w_max, uint64_
         if (keep[k]) { a[tot] = a[mb + k]; b[tot] = b[mb + k]; tot++; }
     a.resize(tot);
     b.resize(tot);
+    g2h::progress("social: %llu community pairs kept", (unsigned long long)(tot - mb));
     auto* g = new g2h_graph;
     g2h::csr_from_edges(n, a, b, *g, omp_get_max_threads());
+    g2h::progress("social: csr built, %llu entries", (unsigned long long)g->ne);
     *out = g;
     return 0;
 }

@@ -14,12 +14,30 @@
 #include <numeric>
 #include <omp.h>
 
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+
 #include "common.hpp"
 
 namespace g2h {
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+
+static bool g_progress = false;
+void progress(const char* fmt, ...) {
+    if (!g_progress) return;
+    static const auto t0 = std::chrono::steady_clock::now();
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "[g2h %8.1fs] ", s);
+    va_list ap;
+    va_start(ap, fmt);
+    std::vfprintf(stderr, fmt, ap);
+    va_end(ap);
+    std::fputc('\n', stderr);
+    std::fflush(stderr);
+}
 
 void csr_from_edges(uint64_t nv, const std::vector<uint32_t>& a, const std::vector<uint32_t>& b,
                     g2h_graph& g, int threads) {
@@ -32,6 +50,7 @@ void csr_from_edges(uint64_t nv, const std::vector<uint32_t>& a, const std::vect
     }
     std::vector<uint64_t> off(nv + 1, 0);
     for (uint64_t v = 0; v < nv; v++) off[v + 1] = off[v] + deg[v];
+    progress("csr: %llu pairs counted", (unsigned long long)m);
     std::vector<uint32_t> tmp(off[nv]);
     std::vector<uint64_t> pos(off.begin(), off.end() - 1);
     for (uint64_t i = 0; i < m; i++) {
@@ -39,6 +58,7 @@ void csr_from_edges(uint64_t nv, const std::vector<uint32_t>& a, const std::vect
         tmp[pos[a[i]]++] = b[i];
         tmp[pos[b[i]]++] = a[i];
     }
+    progress("csr: %llu entries scattered", (unsigned long long)off[nv]);
     std::vector<uint64_t> uniq(nv, 0);
 #pragma omp parallel for schedule(dynamic, 1024) num_threads(threads)
     for (int64_t v = 0; v < (int64_t)nv; v++) {
@@ -63,6 +83,8 @@ using namespace g2h;
 extern "C" {
 
 const char* g2h_last_error(void) { return g_err.c_str(); }
+
+void g2h_set_progress(int on) { g_progress = on != 0; }
 
 int g2h_graph_read_bin(const char* path, g2h_graph** out) {
     FILE* f = std::fopen(path, "rb");
@@ -135,6 +157,7 @@ uint64_t g2h_graph_max_degree(const g2h_graph* g) {
 void g2h_graph_free(g2h_graph* g) { delete g; }
 
 int g2h_graph_preprocess(g2h_graph* g) {
+    progress("preprocess: %llu vertices, %llu entries", (unsigned long long)g->nv, (unsigned long long)g->ne);
     // pass 1: drop self loops and parallel edges (rows are sorted)
     std::vector<uint64_t> nrp(g->nv + 1, 0);
     std::vector<uint32_t> ncols;
@@ -194,6 +217,8 @@ int g2h_graph_kcore_recode(g2h_graph* g, uint32_t* order_out, uint64_t* shell_of
         count += tail;
         level++;
         shells.push_back(count);
+        if (level % 32 == 0) progress("kcore: level %u, %llu of %llu peeled", level, (unsigned long long)count,
+                                      (unsigned long long)V);
     }
     // sort each shell by degree (kcore.cpp:126-151); std::sort on the same
     // sequence with the same comparator reproduces the reference's order

@@ -122,6 +122,9 @@ void g2h_aimd_init(g2h_aimd* a);
 const char* g2h_aimd_adjust(g2h_aimd* a, int overflow);
 
 const char* g2h_last_error(void);
+/* progress lines on stderr from long host steps (generators, CSR build,
+ * preprocess, kcore); off by default */
+void g2h_set_progress(int on);
 
 #ifdef __cplusplus
 }

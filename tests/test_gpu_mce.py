@@ -168,3 +168,23 @@ def test_work_donation(g2, orc, every):
             assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
             assert np.array_equal(got["hist"], ref["hist"])
 
+
+@pytest.mark.parametrize("rpw", [2, 4])
+def test_roots_per_wave(g2, orc, cfg1, rpw):
+    """Batched small roots (lane groups of 32 or 16, roots of degree <= 32 or
+    <= 16) give the oracle's results, alone and with order + split schedules."""
+    print("case: roots per wave", rpw, flush=True)
+    g, _ = cfg1
+    ref = orc.mce(g, per_root=True)
+    with g2.Executor(device=0, mce_roots_per_wave=rpw) as ex:
+        for h in (g, G.gnp(g2, 90, 0.25, 3), G.complete(g2, 17), G.complete(g2, 33), G.cycle(g2, 20)):
+            check(ex, orc, h)
+        ex.transfer(g)
+        rng = np.random.default_rng(rpw)
+        order = rng.permutation(g.nv).astype(np.uint32)
+        split = rng.integers(1, 4, size=g.nv).astype(np.uint32)
+        got = ex.run_mce(per_root=True, order=order, split=split)
+        assert got["total"] == ref["total"] and np.array_equal(got["hist"], ref["hist"])
+        assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+        assert got["stats"]["launches"] >= 2
+
