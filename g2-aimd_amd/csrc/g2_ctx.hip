@@ -48,6 +48,14 @@ static void release(g2_ctx* c, DevBuf& b) {
     b.bytes = 0;
 }
 
+static void apply_tuning_defaults(g2_ctx* c) {
+    if (!c->tune.block_waves) c->tune.block_waves = 4;
+    if (!c->tune.wave_scratch_words) c->tune.wave_scratch_words = 1ull << 16;
+    if (!c->tune.big_waves) c->tune.big_waves = 256;
+    if (!c->tune.mce_small_max) c->tune.mce_small_max = 64;
+    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 256;
+}
+
 }  // namespace g2
 
 using namespace g2;
@@ -97,15 +105,20 @@ int g2_create(g2_ctx** out, int device, uint64_t budget, const g2_tuning* t) {
     if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return G2_EHIP; }
     c->num_cus = p.multiProcessorCount;
     if (t) c->tune = *t;
-    if (!c->tune.block_waves) c->tune.block_waves = 4;
-    if (!c->tune.wave_scratch_words) c->tune.wave_scratch_words = 1ull << 16;
-    if (!c->tune.big_waves) c->tune.big_waves = 256;
-    if (!c->tune.mce_small_max) c->tune.mce_small_max = 64;
-    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 256;
+    apply_tuning_defaults(c);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return G2_EHIP; }
     for (auto& ev : c->ev)
         if (hipEventCreate(&ev) != hipSuccess) { g2_destroy(c); return G2_EHIP; }
     *out = c;
+    return G2_OK;
+}
+
+int g2_set_tuning(g2_ctx* c, const g2_tuning* t) {
+    if (!c || !t) return G2_EINVAL;
+    const uint32_t chunk = c->tune.gm_chunk;
+    c->tune = *t;
+    c->tune.gm_chunk = chunk;  // matching windows are fixed at upload
+    apply_tuning_defaults(c);
     return G2_OK;
 }
 

@@ -100,22 +100,22 @@ __device__ inline uint64_t degree_of(const uint64_t* __restrict__ rp, uint32_t u
 // every reserved record has been taken.  A wave that published records
 // claims any still unclaimed before it can exit, so none is left behind.
 //
-// Coherence without cache maintenance.  Device-scope acquire/release would
-// write back or invalidate an XCD's whole L2, which idle waves polling the
-// queue would do continuously.  Instead every queue word and every record
-// word is accessed with device-scope atomics (coherent across XCDs without
-// touching the rest of the L2), and order between a wave's own accesses
-// comes from waiting for its stores to complete (s_waitcnt) before the next
-// one is issued, and from issuing a load only after the value it depends on
-// has arrived.
+// Coherence with little cache maintenance.  A device-scope acquire
+// invalidates the XCD's L2, which idle waves polling the queue would do
+// continuously; so every queue word and every record word is accessed with
+// device-scope atomics (coherent across XCDs on their own), readers issue a
+// load only after the value it depends on has arrived, and only the writer
+// side orders its stores with a device-scope release (which waits for the
+// earlier stores to complete) -- once per donation and once per wave
+// transition.
 __device__ inline unsigned long long ld_dev(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_dev(unsigned long long* p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// wait until this wave's memory operations have completed
-__device__ inline void drain() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+// this wave's earlier stores complete before its later ones are issued
+__device__ inline void drain() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
 // true when idle waves outnumber the records waiting for them
 __device__ inline bool hungry(const MceArgs& a) {
@@ -156,11 +156,9 @@ __device__ inline void publish_record(const MceArgs& a, uint64_t idx, uint64_t o
     __builtin_amdgcn_wave_barrier();
     if (lane_id() == 0) {
         st_dev(&a.rec_off[idx], off);
-        drain();
-        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&a.dq[DQ_MADE * kDqStride], 1ull);
     }
-    drain();
 }
 
 // add (wrapping) to a queue counter after this wave's earlier stores completed
@@ -168,7 +166,6 @@ __device__ inline void dq_add(const MceArgs& a, int which, unsigned long long v)
     drain();
     __builtin_amdgcn_wave_barrier();
     if (lane_id() == 0) atomicAdd(&a.dq[which * kDqStride], v);
-    drain();
 }
 
 // Next record for an idle wave (counted in DQ_IDLE): its pool offset, or ~0

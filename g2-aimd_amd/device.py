@@ -24,7 +24,7 @@ G2_RUN_ACCOUNT = 1
 G2_RUN_CANONICAL = 2
 
 # every symbol include/g2hip.h declares
-EXPORTS = ("g2_create", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
+EXPORTS = ("g2_create", "g2_set_tuning", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
            "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
            "g2_pinned_free", "g2_plan_local_eligible", "g2_features", "g2_mlp_create",
            "g2_mlp_destroy", "g2_mlp_get_params", "g2_mlp_set_params", "g2_mlp_infer", "g2_mlp_grad",
@@ -76,6 +76,7 @@ def hip_lib() -> C.CDLL:
 
     vp = C.c_void_p
     L.g2_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64, C.POINTER(Tuning)]
+    L.g2_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.g2_destroy.argtypes = [vp]
     L.g2_destroy.restype = None
     L.g2_last_error.argtypes = [vp]
@@ -136,6 +137,11 @@ class Executor:
             raise RuntimeError(f"g2_create failed ({rc}): no usable GPU or bad arguments")
         self.ns = 0
 
+    def set_tuning(self, **tuning) -> None:
+        """New launch tuning for the next runs (the view stays uploaded)."""
+        t = Tuning(**tuning)
+        self._check(hip_lib().g2_set_tuning(self._ctx, C.byref(t)), "g2_set_tuning")
+
     def close(self) -> None:
         if self._ctx:
             hip_lib().g2_destroy(self._ctx)
@@ -169,7 +175,7 @@ class Executor:
         self.ns = len(src)
 
     def run_mce(self, hist_len: int = 256, per_root: bool = False, order=None, split=None,
-                account: bool = False) -> dict:
+                account: bool = False, begin: int = 0, end: int = 0) -> dict:
         """Maximal clique enumeration over the uploaded sources.  `order`
         (permutation of source positions) and `split` (per source: parts its
         first-level children are dealt into) only change the schedule."""
@@ -178,7 +184,7 @@ class Executor:
         prn = np.zeros(max(self.ns, 1), np.uint64) if per_root else None
         st = Stats()
         sched = None
-        if order is not None or split is not None:
+        if order is not None or split is not None or end > begin:
             o = sp = None
             if order is not None:
                 o = np.ascontiguousarray(order, dtype=np.uint32)
@@ -188,7 +194,7 @@ class Executor:
                 sp = np.ascontiguousarray(split, dtype=np.uint32)
                 if len(sp) != self.ns:
                     raise ValueError("split needs one entry per source")
-            sched = Sched(o.ctypes.data if o is not None else None, self.ns, 0, 0,
+            sched = Sched(o.ctypes.data if o is not None else None, self.ns, begin, end,
                           sp.ctypes.data if sp is not None else None)
             self._sched_keep = (o, sp)
         rc = hip_lib().g2_mce_run(self._ctx, C.byref(sched) if sched else None,

@@ -166,3 +166,33 @@ def experiment(g2, dev, scale: int, train_seeds, test_seed: int, epochs: int = 3
     out["speedup_learned_vs_plain"] = out["plain_donate_ms"] / out["learned_donate_ms"]
     out["speedup_learned_vs_plain_nodonate"] = out["plain_nodonate_ms"] / out["learned_nodonate_ms"]
     return out
+
+
+MCE_SHAPES = ({"mce_roots_per_wave": 1, "waves_per_cu": 16}, {"mce_roots_per_wave": 2, "waves_per_cu": 16},
+              {"mce_roots_per_wave": 2, "waves_per_cu": 24}, {"mce_roots_per_wave": 4, "waves_per_cu": 24},
+              {"mce_roots_per_wave": 4, "waves_per_cu": 32})
+
+
+def autotune_mce(ex, shapes=MCE_SHAPES, reps: int = 1, log=print, **run_kw) -> tuple:
+    """Launch shape (roots per wave, resident waves) of the maximal-clique
+    run with the lowest kernel time on the uploaded view; every trial's
+    result is checked equal.  Returns (shape, ms, trials)."""
+    best, best_ms, ref, trials = None, None, None, []
+    for shape in shapes:
+        ex.set_tuning(**shape)
+        ms = []
+        for _ in range(reps + 1):
+            r = ex.run_mce(**run_kw)
+            ms.append(r["stats"]["kernel_ms"])
+        key = (r["total"], r["hist"].tobytes())
+        if ref is None:
+            ref = key
+        elif key != ref:
+            raise RuntimeError("a launch shape changed the result")
+        t = float(np.min(ms[1:]))
+        trials.append({**shape, "ms": t})
+        log(f"shape {shape}: {t:.2f} ms")
+        if best_ms is None or t < best_ms:
+            best, best_ms = shape, t
+    ex.set_tuning(**best)
+    return best, best_ms, trials

@@ -89,6 +89,10 @@ typedef struct g2_sched {
                                      the degree-oriented local-bitmap kernel) */
 
 int  g2_create(g2_ctx** ctx, int device, uint64_t mem_budget_bytes, const g2_tuning* tuning);
+/* Replace the context's launch tuning (0 fields = defaults) between runs, so
+ * a tuner can try shapes without re-uploading the view.  gm_chunk keeps the
+ * value the last upload used.  No counterpart in the reference. */
+int  g2_set_tuning(g2_ctx* ctx, const g2_tuning* tuning);
 void g2_destroy(g2_ctx* ctx);
 const char* g2_last_error(const g2_ctx* ctx);
 
