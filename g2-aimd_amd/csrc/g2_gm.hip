@@ -448,6 +448,19 @@ __global__ __launch_bounds__(256) void gm_batch_kernel(GmArgs a, GmBatchArgs b) 
     if (lane == 0) atomicAdd(&a.ctr[GCTR_NODES], (unsigned long long)nodes_all);
 }
 
+// Largest grid (at most nblocks) whose wave arenas fit the context budget,
+// or, without one, half of the device memory not yet in use (hub rows make
+// max-degree arenas large: R-MAT scale 24 has rows of ~10^5-10^6 entries).
+static uint64_t fit_arenas(g2_ctx* c, uint64_t nblocks, uint64_t per_block) {
+    uint64_t cap = c->budget ? c->budget - std::min(c->budget, c->used - c->scratch.bytes) : 0;
+    if (!c->budget) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) cap = (free_b + c->scratch.bytes) / 2;
+    }
+    while (nblocks > 1 && cap && nblocks * per_block > cap) nblocks /= 2;
+    return nblocks;
+}
+
 int gm_compile(g2_ctx* c, const g2_plan* p) {
     if (p->n < 2 || p->n > G2_MAXQ) return set_err(c, G2_EINVAL, "plan size must be 2..8");
     const uint32_t n = p->n;
@@ -536,7 +549,7 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_st
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
     const uint64_t per_wave = cap * 2 * c->prog.n * 4;
     // shrink the grid until the arenas fit the budget
-    while (nblocks > 1 && c->budget && c->used - c->scratch.bytes + nblocks * bw * per_wave > c->budget) nblocks /= 2;
+    nblocks = fit_arenas(c, nblocks, bw * per_wave);
     int rc;
     if ((rc = ensure(c, c->counters, GCTR_N * 8, "counters"))) return rc;
     if ((rc = ensure(c, c->scratch, nblocks * bw * per_wave, "wave arenas"))) return rc;
@@ -650,7 +663,7 @@ int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* c
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * wpc / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
     const uint64_t per_wave = cap * 2 * levels * 4;
-    while (nblocks > 1 && c->budget && c->used - c->scratch.bytes + nblocks * bw * per_wave > c->budget) nblocks /= 2;
+    nblocks = fit_arenas(c, nblocks, bw * per_wave);
     if ((rc = ensure(c, c->counters, GCTR_N * 8, "counters"))) return rc;
     if ((rc = ensure(c, c->scratch, nblocks * bw * per_wave, "wave arenas"))) return rc;
     if ((rc = ensure(c, c->batch_progs, (uint64_t)nq * sizeof(GmProgram), "batch programs"))) return rc;

@@ -100,22 +100,27 @@ __device__ inline uint64_t degree_of(const uint64_t* __restrict__ rp, uint32_t u
 // every reserved record has been taken.  A wave that published records
 // claims any still unclaimed before it can exit, so none is left behind.
 //
-// Coherence with little cache maintenance.  A device-scope acquire
-// invalidates the XCD's L2, which idle waves polling the queue would do
-// continuously; so every queue word and every record word is accessed with
-// device-scope atomics (coherent across XCDs on their own), readers issue a
-// load only after the value it depends on has arrived, and only the writer
-// side orders its stores with a device-scope release (which waits for the
-// earlier stores to complete) -- once per donation and once per wave
-// transition.
+// Coherence without cache maintenance.  A device-scope acquire invalidates
+// the XCD's L2 and a device-scope release writes it back, which idle waves
+// polling the queue, and donors, would do over and over.  Instead every
+// queue word and every record word is accessed with device-scope atomics
+// (coherent across XCDs on their own); a writer waits for its earlier
+// stores to complete (s_waitcnt) before it issues the store that publishes
+// them, and a reader issues a load only after the value it depends on has
+// arrived.
 __device__ inline unsigned long long ld_dev(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_dev(unsigned long long* p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// this wave's earlier stores complete before its later ones are issued
-__device__ inline void drain() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// this wave's earlier memory operations complete before its later ones are
+// issued (the signal fences keep the compiler from moving accesses across)
+__device__ inline void drain() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // true when idle waves outnumber the records waiting for them
 __device__ inline bool hungry(const MceArgs& a) {
@@ -156,7 +161,8 @@ __device__ inline void publish_record(const MceArgs& a, uint64_t idx, uint64_t o
     __builtin_amdgcn_wave_barrier();
     if (lane_id() == 0) {
         st_dev(&a.rec_off[idx], off);
-        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        drain();
+        __hip_atomic_store(&a.rec_ready[idx], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&a.dq[DQ_MADE * kDqStride], 1ull);
     }
 }
@@ -204,7 +210,8 @@ constexpr int kMaxBlockWaves = 4;  // __launch_bounds__(256)
 struct SmallLds {
     uint32_t C[kWave];         // the root's neighbourhood, sorted
     uint32_t pref[kWave];      // exclusive prefix of the streamed row lengths per P0 vertex
-    uint64_t start[kWave];     // first streamed cols index per P0 vertex
+    uint64_t start[kWave];     // first cols index of the row slice per P0 vertex
+    uint32_t rlen[kWave];      // slice length; bit 31: look C up in the slice instead of streaming it
     uint64_t adjP[kWave];      // N(C[k]) & P0, bit j = P0[j]
     uint64_t adjX[kWave];      // N(P0[j]) & X0, bit k = C[k]
 };
@@ -248,7 +255,7 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
     L.adjP[lane] = 0;
     L.adjX[lane] = 0;
     wave_sync();
-    uint32_t len = 0;
+    uint32_t len = 0;  // work items of P0 vertex `lane`
     if (lane < p0) {
         const uint32_t p = L.C[s + lane];
         uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
@@ -257,7 +264,12 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
             e1 = lower_bound(a.cols, e0, e1, L.C[d - 1] + 1);
         }
         L.start[lane] = e0;
-        len = (uint32_t)(e1 - e0);
+        const uint32_t n = (uint32_t)(e1 - e0);
+        // stream the slice (n items), or binary-search each of the d positions
+        // in it when that is cheaper (hub rows)
+        const bool rev = (uint64_t)d * (1 + bitlen(n)) < n;
+        L.rlen[lane] = rev ? (n | 0x80000000u) : n;
+        len = rev ? d : n;
     }
     uint32_t incl = len;
 #pragma unroll
@@ -276,9 +288,18 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
             if (L.pref[mid] <= f) lo2 = mid; else hi2 = mid;
         }
         const uint32_t j = lo2;
-        const uint32_t w = a.cols[L.start[j] + (f - L.pref[j])];
-        const uint32_t k = lds_lower_bound(L.C, 0, d, w);
-        if (k < d && L.C[k] == w) {
+        const uint32_t rl = L.rlen[j], item = f - L.pref[j];
+        uint32_t k;
+        bool hit;
+        if (rl & 0x80000000u) {  // position `item` of C looked up in the slice
+            k = item;
+            hit = contains(a.cols, L.start[j], L.start[j] + (rl & 0x7fffffffu), L.C[k]);
+        } else {
+            const uint32_t w = a.cols[L.start[j] + item];
+            k = lds_lower_bound(L.C, 0, d, w);
+            hit = k < d && L.C[k] == w;
+        }
+        if (hit) {
             if (k < s) {
                 atomicOr((unsigned long long*)&L.adjX[j], 1ull << k);
                 atomicOr((unsigned long long*)&L.adjP[k], 1ull << j);
@@ -777,7 +798,10 @@ __global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
                 e1 = lower_bound(a.cols, e0, e1, L.C[gbase + d - 1] + 1);
             }
             L.start[lane] = e0;
-            len = (uint32_t)(e1 - e0);
+            const uint32_t n = (uint32_t)(e1 - e0);
+            const bool rev = (uint64_t)d * (1 + bitlen(n)) < n;
+            L.rlen[lane] = rev ? (n | 0x80000000u) : n;
+            len = rev ? d : n;
         }
         uint32_t incl = len;
 #pragma unroll
@@ -795,9 +819,18 @@ __global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
                 if (L.pref[gbase + mid] <= f) lo2 = mid; else hi2 = mid;
             }
             const uint32_t j = lo2;
-            const uint32_t w = a.cols[L.start[gbase + j] + (f - L.pref[gbase + j])];
-            const uint32_t k = lds_lower_bound(L.C + gbase, 0, d, w);
-            if (k < d && L.C[gbase + k] == w) {
+            const uint32_t rl = L.rlen[gbase + j], item = f - L.pref[gbase + j];
+            uint32_t k;
+            bool hit;
+            if (rl & 0x80000000u) {
+                k = item;
+                hit = contains(a.cols, L.start[gbase + j], L.start[gbase + j] + (rl & 0x7fffffffu), L.C[gbase + k]);
+            } else {
+                const uint32_t w = a.cols[L.start[gbase + j] + item];
+                k = lds_lower_bound(L.C + gbase, 0, d, w);
+                hit = k < d && L.C[gbase + k] == w;
+            }
+            if (hit) {
                 if (k < s) {
                     atomicOr((unsigned long long*)&L.adjX[gbase + j], 1ull << k);
                     atomicOr((unsigned long long*)&L.adjP[gbase + k], 1ull << j);
