@@ -55,6 +55,7 @@ struct WaveState {
     uint64_t len[G2_MAXQ];
     uint64_t idx[G2_MAXQ];
     uint64_t prelen[G2_MAXQ];
+    uint64_t olo[G2_MAXQ], ohi[G2_MAXQ];  // narrowed rows of the membership tests (wave-uniform)
 };
 
 __device__ inline Range cond_range(const WaveState& S, uint32_t n, const uint32_t* op, const uint32_t* pos) {
@@ -84,7 +85,7 @@ __device__ inline uint64_t store_bytes(const GmStep& st, uint32_t id, uint32_t n
 // out != nullptr the survivors are appended (sorted) and their number is
 // returned; otherwise they are only counted into *lane_count (per lane).
 template <bool ACCOUNT>
-__device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn, const uint32_t* bn,
+__device__ uint64_t filter_row(const GmArgs& a, WaveState& S, uint32_t nbn, const uint32_t* bn,
                                uint32_t parent, uint64_t row_lo, uint64_t row_hi, Range rg,
                                uint32_t ncond, const uint32_t* op, const uint32_t* pos, uint32_t* out,
                                uint64_t* lane_count, uint64_t* bytes, uint64_t* npass_out) {
@@ -93,7 +94,8 @@ __device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn
     uint64_t s0 = lower_bound(cols, row_lo, row_hi, rg.lo);
     uint64_t s1 = rg.hi == 0xffffffffu ? row_hi : lower_bound(cols, s0, row_hi, rg.hi);
     // narrowed rows of the other back neighbours
-    uint64_t olo[G2_MAXQ], ohi[G2_MAXQ];
+    // narrowed rows of the other back neighbours, kept in LDS: the values are
+    // wave-uniform and an indexed register array would spill
     uint32_t no = 0;
     uint64_t bcost = 0;
     for (uint32_t k = 0; k < nbn; k++) {
@@ -103,10 +105,13 @@ __device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn
         if (ACCOUNT) bcost += cost_isect(1, h - l);
         l = lower_bound(cols, l, h, rg.lo);
         if (rg.hi != 0xffffffffu) h = lower_bound(cols, l, h, rg.hi);
-        olo[no] = l;
-        ohi[no] = h;
+        if (lane == 0) {
+            S.olo[no] = l;
+            S.ohi[no] = h;
+        }
         no++;
     }
+    wave_sync();
     if (!out && no == 0) {
         // counting with no membership test: the whole slice passes except the
         // entries a NON_EQUAL condition names (matched values are distinct)
@@ -128,7 +133,7 @@ __device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn
             w = cols[e];
             pass = neq_ok(S, ncond, op, pos, w);
             ok = pass;
-            for (uint32_t k = 0; k < no && ok; k++) ok = contains(cols, olo[k], ohi[k], w);
+            for (uint32_t k = 0; k < no && ok; k++) ok = contains(cols, S.olo[k], S.ohi[k], w);
         }
         if (ACCOUNT) npass += pass;
         const uint64_t m = ballot(ok);
@@ -149,22 +154,25 @@ __device__ uint64_t filter_row(const GmArgs& a, const WaveState& S, uint32_t nbn
 
 // count or collect the after-filtered entries of a pre list for position id
 template <bool ACCOUNT>
-__device__ uint64_t filter_pre(const GmArgs& a, const WaveState& S, const GmStep& st, const uint32_t* pre,
+__device__ uint64_t filter_pre(const GmArgs& a, WaveState& S, const GmStep& st, const uint32_t* pre,
                                uint64_t plen, uint32_t* out, uint64_t* lane_count, uint64_t* bytes) {
     const uint32_t lane = lane_id();
     const Range rg = cond_range(S, st.nacond, st.acop, st.acpos);
     // pre is sorted: restrict to the value range
     const uint64_t p0 = lower_bound(pre, 0, plen, rg.lo);
     const uint64_t p1 = rg.hi == 0xffffffffu ? plen : lower_bound(pre, p0, plen, rg.hi);
-    uint64_t alo[G2_MAXQ], ahi[G2_MAXQ];
     for (uint32_t k = 0; k < st.nafter; k++) {
         const uint32_t m = S.M[st.after[k]];
-        alo[k] = a.rp[m];
-        ahi[k] = a.rp[m + 1];
-        if (ACCOUNT && lane == 0) *bytes += cost_isect(plen, ahi[k] - alo[k]);
-        alo[k] = lower_bound(a.cols, alo[k], ahi[k], rg.lo);
-        if (rg.hi != 0xffffffffu) ahi[k] = lower_bound(a.cols, alo[k], ahi[k], rg.hi);
+        uint64_t l = a.rp[m], h = a.rp[m + 1];
+        if (ACCOUNT && lane == 0) *bytes += cost_isect(plen, h - l);
+        l = lower_bound(a.cols, l, h, rg.lo);
+        if (rg.hi != 0xffffffffu) h = lower_bound(a.cols, l, h, rg.hi);
+        if (lane == 0) {
+            S.olo[k] = l;
+            S.ohi[k] = h;
+        }
     }
+    wave_sync();
     if (!out && st.nafter == 0) {  // counting, no after rows: closed form as in filter_row
         uint64_t cnt = p1 - p0;
         for (uint32_t k = 0; k < st.nacond; k++) {
@@ -182,7 +190,7 @@ __device__ uint64_t filter_pre(const GmArgs& a, const WaveState& S, const GmStep
         if (e < p1) {
             w = pre[e];
             ok = neq_ok(S, st.nacond, st.acop, st.acpos, w);
-            for (uint32_t k = 0; k < st.nafter && ok; k++) ok = contains(a.cols, alo[k], ahi[k], w);
+            for (uint32_t k = 0; k < st.nafter && ok; k++) ok = contains(a.cols, S.olo[k], S.ohi[k], w);
         }
         const uint64_t m = ballot(ok);
         if (out) {
