@@ -53,7 +53,7 @@ static void apply_tuning_defaults(g2_ctx* c) {
     if (!c->tune.wave_scratch_words) c->tune.wave_scratch_words = 1ull << 16;
     if (!c->tune.big_waves) c->tune.big_waves = 256;
     if (!c->tune.mce_small_max) c->tune.mce_small_max = 64;
-    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 1024;
+    if (!c->tune.mce_donate_every) c->tune.mce_donate_every = 256;
 }
 
 }  // namespace g2
