@@ -205,6 +205,7 @@ __device__ inline uint64_t claim_record(const MceArgs& a, uint64_t total_tasks) 
 }
 
 constexpr int kMaxBlockWaves = 4;  // __launch_bounds__(256)
+constexpr int kScoreWords = 2;     // candidate words scored per round (arena path)
 
 // LDS of one wave for the register-resident path
 struct SmallLds {
@@ -607,43 +608,45 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                         else atomicAdd(&a.hist[sz < a.hist_len ? sz : a.hist_len - 1], 1ull);
                     }
                 } else {
-                    // pivot over X0 candidates, then P0 candidates (position order)
+                    // pivot over X0 candidates, then P0 candidates (position order).
+                    // Candidate words are taken kScoreWords at a time so the row
+                    // loads of their candidates are in flight together.
                     uint64_t best = 0;
-                    for (uint64_t c0 = 0; c0 < WX; c0 += kWave) {
-                        const uint64_t wx = (c0 + lane < WX) ? XX[c0 + lane] : 0;
-                        uint64_t nz = ballot(wx != 0);
+                    auto score_words = [&](uint64_t wsrc, uint64_t nz, uint64_t kbase) {
                         while (nz) {
-                            const uint32_t wi = __builtin_ctzll(nz);
-                            nz &= nz - 1;
-                            const uint64_t word = read_lane(wx, wi);
-                            if ((word >> lane) & 1) {
-                                const uint64_t k = (c0 + wi) * 64 + lane;
-                                const uint64_t* row = adjP + k * W;
-                                uint64_t sc = 0;
-                                for (uint64_t i = 0; i < W; i++) sc += __popcll(row[i] & read_lane(pw, (uint32_t)i));
-                                const uint64_t key = (sc << 32) | k;
+                            uint32_t wq[kScoreWords];
+                            uint64_t on[kScoreWords], kq[kScoreWords], sc[kScoreWords];
+#pragma unroll
+                            for (int q = 0; q < kScoreWords; q++) {
+                                wq[q] = nz ? (uint32_t)__builtin_ctzll(nz) : 64u;
+                                nz &= nz ? nz - 1 : 0;
+                                const uint64_t word = wq[q] < 64 ? read_lane(wsrc, wq[q]) : 0;
+                                on[q] = (word >> lane) & 1;
+                                kq[q] = kbase + (uint64_t)(wq[q] & 63u) * 64 + lane;
+                                sc[q] = 0;
+                            }
+                            for (uint64_t i = 0; i < W; i++) {
+                                const uint64_t pi = read_lane(pw, (uint32_t)i);
+#pragma unroll
+                                for (int q = 0; q < kScoreWords; q++)
+                                    if (on[q]) sc[q] += __popcll(adjP[kq[q] * W + i] & pi);
+                            }
+#pragma unroll
+                            for (int q = 0; q < kScoreWords; q++) {
+                                if (!on[q]) continue;
+                                const uint64_t key = (sc[q] << 32) | kq[q];
                                 best = key > best ? key : best;
-                                if (ACCOUNT && own) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
+                                if (ACCOUNT && own) my_bytes += cost_isect(np, degree_of(a.rp, C[kq[q]]));
                             }
                         }
+                    };
+                    for (uint64_t c0 = 0; c0 < WX; c0 += kWave) {
+                        const uint64_t wx = (c0 + lane < WX) ? XX[c0 + lane] : 0;
+                        score_words(wx, ballot(wx != 0), c0 * 64);
                     }
                     {
                         const uint64_t uw = pw | xpw;
-                        uint64_t nz = ballot(uw != 0);
-                        while (nz) {
-                            const uint32_t wi = __builtin_ctzll(nz);
-                            nz &= nz - 1;
-                            const uint64_t word = read_lane(uw, wi);
-                            if ((word >> lane) & 1) {
-                                const uint64_t k = s + (uint64_t)wi * 64 + lane;
-                                const uint64_t* row = adjP + k * W;
-                                uint64_t sc = 0;
-                                for (uint64_t i = 0; i < W; i++) sc += __popcll(row[i] & read_lane(pw, (uint32_t)i));
-                                const uint64_t key = (sc << 32) | k;
-                                best = key > best ? key : best;
-                                if (ACCOUNT && own) my_bytes += cost_isect(np, degree_of(a.rp, C[k]));
-                            }
-                        }
+                        score_words(uw, ballot(uw != 0), s);
                     }
                     best = wave_max(best);
                     const uint64_t pivot = best & 0xffffffffull;
