@@ -640,9 +640,57 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
                             }
                         }
                     };
-                    for (uint64_t c0 = 0; c0 < WX; c0 += kWave) {
-                        const uint64_t wx = (c0 + lane < WX) ? XX[c0 + lane] : 0;
-                        score_words(wx, ballot(wx != 0), c0 * 64);
+                    const uint64_t npw = wave_sum(__popcll(pw));  // |P|
+                    if (!ACCOUNT && WX <= kWave && npw < 32) {
+                        // Few P members: score all X candidates at once from the
+                        // columns of adjX (score(x) = sum over j in P of bit x of
+                        // adjX[j], the same number as popc(adjP[x] & P) since the
+                        // bitsets are symmetric).  Lane i adds word i of each
+                        // member's row into 5 bit-sliced counters, then keeps the
+                        // candidates of its word with the largest count.
+                        const uint64_t xw = lane < WX ? XX[lane] : 0;
+                        uint64_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+                        for (uint32_t wi = 0; wi < W; wi++) {
+                            uint64_t word = read_lane(pw, wi);
+                            while (word) {
+                                uint64_t r[4];
+#pragma unroll
+                                for (int q = 0; q < 4; q++) {
+                                    const uint64_t j = word ? (uint64_t)wi * 64 + __builtin_ctzll(word) : ~0ull;
+                                    word &= word ? word - 1 : 0;
+                                    r[q] = (j != ~0ull && lane < WX) ? adjX[j * WX + lane] : 0;
+                                }
+#pragma unroll
+                                for (int q = 0; q < 4; q++) {
+                                    uint64_t c = r[q], t;
+                                    t = b0 & c; b0 ^= c; c = t;
+                                    t = b1 & c; b1 ^= c; c = t;
+                                    t = b2 & c; b2 ^= c; c = t;
+                                    t = b3 & c; b3 ^= c; c = t;
+                                    b4 ^= c;
+                                }
+                            }
+                        }
+                        uint64_t mm = xw, sc = 0;
+                        const uint64_t planes[5] = {b0, b1, b2, b3, b4};
+#pragma unroll
+                        for (int b = 4; b >= 0; b--) {
+                            const uint64_t t = mm & planes[b];
+                            if (t) {
+                                mm = t;
+                                sc |= 1ull << b;
+                            }
+                        }
+                        if (xw) {
+                            const uint64_t k = (uint64_t)lane * 64 + (63 - __builtin_clzll(mm));
+                            const uint64_t key = (sc << 32) | k;
+                            best = key > best ? key : best;
+                        }
+                    } else {
+                        for (uint64_t c0 = 0; c0 < WX; c0 += kWave) {
+                            const uint64_t wx = (c0 + lane < WX) ? XX[c0 + lane] : 0;
+                            score_words(wx, ballot(wx != 0), c0 * 64);
+                        }
                     }
                     {
                         const uint64_t uw = pw | xpw;
