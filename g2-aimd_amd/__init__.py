@@ -143,6 +143,8 @@ def host_lib() -> C.CDLL:
                                        C.c_uint32 * (2 * G2_MAXQ * G2_MAXQ)]
     L.g2h_query_automorphisms.argtypes = [C.c_uint32, qa]
     L.g2h_query_automorphisms.restype = C.c_uint64
+    L.g2h_query_canonical.argtypes = [C.c_uint32, qa]
+    L.g2h_query_canonical.restype = C.c_uint64
     L.g2h_plan_build.argtypes = [C.c_uint32, qa, C.POINTER(G2Plan)]
     L.g2h_query_random.argtypes = [C.c_uint32, C.c_double, C.c_uint64, qa]
     L.g2h_view_materialize.argtypes = [vp, u32p, C.c_uint64, C.c_int, C.c_uint64, u64p, u32p,
@@ -317,6 +319,11 @@ def query_conditions(qadj):
 
 def query_automorphisms(qadj) -> int:
     return int(host_lib().g2h_query_automorphisms(len(qadj), _qarr(qadj)))
+
+
+def query_canonical(qadj) -> int:
+    """isomorphism-invariant code of a pattern (equal <=> isomorphic)"""
+    return int(host_lib().g2h_query_canonical(len(qadj), _qarr(qadj)))
 
 
 def query_random(n: int, density: float, seed: int):

@@ -49,3 +49,17 @@ def cost_key(qadj) -> tuple:
 
 def heavy_first(queries) -> np.ndarray:
     return np.array(sorted(range(len(queries)), key=lambda i: cost_key(queries[i])), dtype=np.uint32)
+
+
+def isomorphism_classes(g2, queries):
+    """(representatives, class index of every query): isomorphic patterns
+    have the same count (injective homomorphisms / |Aut|), so a batch counts
+    each class once and hands the count to every member"""
+    codes, reps, cls = {}, [], []
+    for q in queries:
+        c = g2.query_canonical(q)
+        if c not in codes:
+            codes[c] = len(reps)
+            reps.append(q)
+        cls.append(codes[c])
+    return reps, np.array(cls, dtype=np.int64)

@@ -240,6 +240,25 @@ uint64_t g2h_query_automorphisms(uint32_t n, const uint32_t qadj[G2_MAXQ]) {
     return automorphisms(rows_of(n, qadj)).size();
 }
 
+// Isomorphism-invariant code: the smallest upper-triangle adjacency bit
+// string over all relabellings (n <= 8, so at most 8! = 40320 of them), with
+// n in the top byte.  Equal codes <=> isomorphic patterns.
+uint64_t g2h_query_canonical(uint32_t n, const uint32_t qadj[G2_MAXQ]) {
+    if (n < 1 || n > G2_MAXQ) return 0;
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    uint64_t best = ~0ull;
+    do {
+        uint64_t code = 0;
+        uint32_t bit = 0;
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t j = i + 1; j < n; j++, bit++)
+                if (qadj[perm[i]] >> perm[j] & 1) code |= 1ull << bit;
+        best = std::min(best, code);
+    } while (std::next_permutation(perm.begin(), perm.end()));
+    return ((uint64_t)n << 56) | best;
+}
+
 int g2h_plan_build(uint32_t n, const uint32_t qadj[G2_MAXQ], g2_plan* P) {
     if (n < 2 || n > G2_MAXQ) { g2h::set_error("pattern size out of range"); return -1; }
     const Rows r = rows_of(n, qadj);
