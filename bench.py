@@ -309,9 +309,11 @@ def run_batch(args, g2, dev):
         ex.transfer(g)
         solo = []
         t0 = time.perf_counter()
-        for q in adm:
+        for i, q in enumerate(adm):
             ex.set_plan(plans[q])
             solo.append(ex.run_gm(canonical=True)["count"])
+            if (i + 1) % 32 == 0:
+                log(f"serial: {i + 1} of {len(adm)} queries, {time.perf_counter() - t0:.1f}s")
         serial_ms = (time.perf_counter() - t0) * 1e3
     solo = np.array(solo, np.uint64)
     if not np.array_equal(solo, counts) or not np.array_equal(solo, rall["counts"]):
