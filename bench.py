@@ -276,19 +276,36 @@ def run_batch(args, g2, dev):
     log(f"{len(qs)} queries in {len(reps)} isomorphism classes")
     with dev.Executor(device=0) as ex:
         ex.transfer(g)
+        # admission pilot (untimed): classes in chunks of 8, heaviest last, each
+        # chunk bounded by the node budget; classes not reached before the
+        # pilot's time guard count as not admitted
+        adm_flag = np.zeros(len(reps), bool)
+        tried = np.zeros(len(reps), bool)
         t0 = time.perf_counter()
-        shape = ex.autotune(rplans, sample_queries=args.tune_sample, seed=1, node_budget=args.node_budget)
+        light_first = rorder[::-1]
+        for c0 in range(0, len(reps), 8):
+            if time.perf_counter() - t0 > args.pilot_seconds:
+                log("pilot time guard reached")
+                break
+            chunk = light_first[c0:c0 + 8]
+            r = ex.run_gm_batch([rplans[i] for i in chunk], node_budget=args.node_budget, window=256)
+            adm_flag[chunk] = ~r["over_budget"]
+            tried[chunk] = True
+            log(f"pilot: classes {c0}-{c0 + len(chunk) - 1}: {int((~r['over_budget']).sum())} within budget, "
+                f"{r['stats']['kernel_ms']:.0f} ms")
+        pilot_s = time.perf_counter() - t0
+        adm_reps = np.nonzero(adm_flag)[0]
+        adm = np.nonzero(adm_flag[cls])[0]                # admitted queries
+        log(f"{len(adm)} of {len(qs)} queries ({len(adm_reps)} of {len(reps)} classes, {int(tried.sum())} tried) "
+            f"within the budget of {args.node_budget} extensions; pilot {pilot_s:.1f}s")
+        ap = [rplans[i] for i in adm_reps]
+        ao = B.heavy_first([reps[i] for i in adm_reps])
+        t0 = time.perf_counter()
+        shape = ex.autotune(ap, sample_queries=args.tune_sample, seed=1)
         tune_s = time.perf_counter() - t0
         log(f"tuned shape {shape} in {tune_s:.1f}s")
         kw = {k: shape[k] for k in ("window", "roots_per_task", "waves_per_cu")}
-        r = ex.run_gm_batch(rplans, order=rorder, node_budget=args.node_budget, **kw)  # warm-up, admission
-        radm = ~r["over_budget"]
-        adm = np.nonzero(radm[cls])[0]                    # admitted queries
-        adm_reps = np.nonzero(radm)[0]
-        log(f"{len(adm)} of {len(qs)} queries ({len(adm_reps)} of {len(reps)} classes) within the budget "
-            f"of {args.node_budget} extensions")
-        ap = [rplans[i] for i in adm_reps]
-        ao = B.heavy_first([reps[i] for i in adm_reps])
+        ex.run_gm_batch(ap, order=ao, **kw)  # warm-up
         ts = []
         for i in range(args.steps):
             t0 = time.perf_counter()
@@ -328,7 +345,8 @@ def run_batch(args, g2, dev):
                                f"on R-MAT scale {args.scale}, edge factor 16",
                    "vertices": g.nv, "undirected_edges": g.ne // 2, "classes": len(reps),
                    "admitted_queries": len(adm), "admitted_classes": len(adm_reps),
-                   "node_budget": args.node_budget, "shape": shape, "tune_seconds": tune_s},
+                   "node_budget": args.node_budget, "classes_tried": int(tried.sum()),
+                   "pilot_seconds": pilot_s, "shape": shape, "tune_seconds": tune_s},
         "batched_every_query": {"value": len(adm) / (all_ms / 1e3), "ms": all_ms},
         "serial": {"value": len(adm) / (serial_ms / 1e3), "ms": serial_ms},
         "speedup_vs_serial": serial_ms / batch_ms,
@@ -364,7 +382,8 @@ def main():
                          "random patterns, batched vs serial launches (use --scale 24)")
     ap.add_argument("--queries", type=int, default=512)
     ap.add_argument("--tune-sample", type=int, default=32)
-    ap.add_argument("--node-budget", type=int, default=50_000_000)
+    ap.add_argument("--node-budget", type=int, default=1_000_000_000)
+    ap.add_argument("--pilot-seconds", type=float, default=300.0)
     ap.add_argument("--social-n", type=int, default=65_000_000)
     ap.add_argument("--train-n", type=int, default=2_000_000)
     ap.add_argument("--train-graphs", type=int, default=4)
