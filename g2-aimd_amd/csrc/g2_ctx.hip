@@ -128,7 +128,7 @@ void g2_destroy(g2_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
                       &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans,
-                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate})
+                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate, &c->taskbuild})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -198,15 +198,8 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
         return set_err(c, G2_EINVAL, "empty schedule range");
     if (sched && (sched->begin > sched->end || sched->end > c->ns))
         return set_err(c, G2_EINVAL, "schedule range outside the sources");
-    if (sched && sched->order) {
-        if (sched->n != c->ns) return set_err(c, G2_EINVAL, "schedule length != source count");
-        std::vector<uint8_t> seen(c->ns, 0);
-        for (uint64_t i = 0; i < c->ns; i++) {
-            const uint32_t o = sched->order[i];
-            if (o >= c->ns || seen[o]) return set_err(c, G2_EINVAL, "schedule is not a permutation of the sources");
-            seen[o] = 1;
-        }
-    }
+    // the permutation itself is checked on the device, while the task list is built
+    if (sched && sched->order && sched->n != c->ns) return set_err(c, G2_EINVAL, "schedule length != source count");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     return mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);

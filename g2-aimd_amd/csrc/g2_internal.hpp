@@ -81,6 +81,7 @@ struct g2_ctx {
     g2::DevBuf order;
     g2::DevBuf spans;
     g2::DevBuf donate;            // MCE donation queue (counters, record table, pool)
+    g2::DevBuf taskbuild;         // MCE task-list construction scratch
 
     // matching
     bool has_plan = false;

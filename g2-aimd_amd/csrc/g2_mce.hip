@@ -27,6 +27,8 @@
 // with a larger arena (never dropped).
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "g2_device.hpp"
 #include "g2_internal.hpp"
 
@@ -1004,6 +1006,147 @@ __global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
         if (lhist[i]) atomicAdd(&a.hist[i < a.hist_len ? i : a.hist_len - 1], lhist[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Task list, built on the device.  Schedule position t holds source
+// si = order[t]; it goes to the lane-group list (G > 0, not split, degree
+// <= G) or, as its k part codes si | j << 32 | k << 48, to the main list.
+// Exclusive scans of the per-position counts keep both lists in schedule
+// order.  A full order is also checked to be a permutation of the sources.
+// ---------------------------------------------------------------------------
+struct TaskBuild {
+    const uint32_t* __restrict__ order;  // nullable: identity
+    const uint32_t* __restrict__ split;  // nullable: every root whole
+    const uint32_t* __restrict__ src;
+    const uint32_t* __restrict__ deg;    // degree by vertex id
+    uint64_t b, n, ns;                   // positions [b, b + n) of ns sources
+    uint32_t G;
+    unsigned long long* cnt_main;        // n: part codes of position t
+    unsigned int* cnt_small;             // n: 1 when position t is a lane-group root
+};
+
+__device__ inline void task_of(const TaskBuild& a, uint64_t i, uint32_t* si, uint32_t* k, bool* small) {
+    const uint64_t t = a.b + i;
+    *si = a.order ? a.order[t] : (uint32_t)t;
+    *k = a.split ? max(1u, min(a.split[*si], 0xffffu)) : 1u;
+    *small = a.G && *k == 1 && a.deg[a.src[*si]] <= a.G;
+}
+
+__global__ __launch_bounds__(256) void task_count_kernel(TaskBuild a) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t si, k;
+        bool small;
+        task_of(a, i, &si, &k, &small);
+        a.cnt_main[i] = small ? 0 : k;
+        a.cnt_small[i] = small ? 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void task_fill_kernel(TaskBuild a, const unsigned long long* off_main,
+                                                        const unsigned int* off_small, uint64_t* out, uint64_t main_total) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t si, k;
+        bool small;
+        task_of(a, i, &si, &k, &small);
+        if (small) {
+            out[main_total + off_small[i]] = si;
+        } else {
+            for (uint32_t j = 0; j < k; j++)
+                out[off_main[i] + j] = (uint64_t)si | ((uint64_t)j << 32) | ((uint64_t)k << 48);
+        }
+    }
+}
+
+// every entry of order (all ns of them) in range and seen once
+__global__ __launch_bounds__(256) void perm_check_kernel(const uint32_t* __restrict__ order, uint64_t ns,
+                                                         unsigned int* seen, unsigned int* bad) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = order[i];
+        if (o >= ns) {
+            atomicOr(bad, 1u);
+            continue;
+        }
+        const unsigned int m = 1u << (o & 31);
+        if (atomicOr(&seen[o >> 5], m) & m) atomicOr(bad, 1u);
+    }
+}
+
+// Build the main and lane-group task lists in c->order (main codes first).
+static int build_tasks(g2_ctx* c, const g2_sched* sched, uint32_t G, uint64_t* nmain, uint64_t* nsmall) {
+    const bool ranged = sched && sched->end > sched->begin;
+    const uint64_t b = ranged ? sched->begin : 0, en = ranged ? sched->end : c->ns, n = en - b;
+    const bool has_order = sched && sched->order, has_split = sched && sched->split;
+    const uint64_t ns = c->ns;
+    const uint64_t seen_words = (ns + 31) / 32;
+    if (n > 0x7fffffffull) return set_err(c, G2_EINVAL, "schedules of more than 2^31-1 positions are not supported");
+    // layout: order u32[ns] | split u32[ns] | cnt_main u64[n] | off_main u64[n] |
+    //         cnt_small u32[n] | off_small u32[n] | seen u32[seen_words] | bad u32 | scan temp
+    size_t t1 = 0, t2 = 0;
+    hipError_t e;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, t1, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                                               (int)std::max<uint64_t>(n, 1), c->stream)))
+        return hip_err(c, e, "scan size");
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, t2, (unsigned int*)nullptr, (unsigned int*)nullptr,
+                                               (int)std::max<uint64_t>(n, 1), c->stream)))
+        return hip_err(c, e, "scan size");
+    const uint64_t temp = std::max<uint64_t>(t1, t2);
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    const uint64_t o_split = al(ns * 4), o_cm = o_split + al(ns * 4), o_om = o_cm + al(n * 8), o_cs = o_om + al(n * 8),
+                   o_os = o_cs + al(n * 4), o_seen = o_os + al(n * 4), o_bad = o_seen + al(seen_words * 4),
+                   o_tmp = o_bad + 256, total = o_tmp + al(temp);
+    int rc;
+    if ((rc = ensure(c, c->taskbuild, total, "task-list scratch"))) return rc;
+    char* base = (char*)c->taskbuild.ptr;
+    auto* d_order = (uint32_t*)base;
+    auto* d_split = (uint32_t*)(base + o_split);
+    auto* cnt_main = (unsigned long long*)(base + o_cm);
+    auto* off_main = (unsigned long long*)(base + o_om);
+    auto* cnt_small = (unsigned int*)(base + o_cs);
+    auto* off_small = (unsigned int*)(base + o_os);
+    auto* seen = (unsigned int*)(base + o_seen);
+    auto* bad = (unsigned int*)(base + o_bad);
+    if (has_order && (e = hipMemcpyAsync(d_order, sched->order, ns * 4, hipMemcpyHostToDevice, c->stream)))
+        return hip_err(c, e, "H2D order");
+    if (has_split && (e = hipMemcpyAsync(d_split, sched->split, ns * 4, hipMemcpyHostToDevice, c->stream)))
+        return hip_err(c, e, "H2D split");
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((std::max(n, ns) + 255) / 256, 8192));
+    if (has_order) {
+        if ((e = hipMemsetAsync(seen, 0, seen_words * 4 + 4, c->stream))) return hip_err(c, e, "memset seen");
+        if ((e = hipMemsetAsync(bad, 0, 4, c->stream))) return hip_err(c, e, "memset flag");
+        hipLaunchKernelGGL(perm_check_kernel, dim3(grid), dim3(256), 0, c->stream, d_order, ns, seen, bad);
+    }
+    TaskBuild a{has_order ? d_order : nullptr, has_split ? d_split : nullptr, (const uint32_t*)c->src.ptr,
+                (const uint32_t*)c->deg.ptr, b, n, ns, G, cnt_main, cnt_small};
+    if (n) {
+        hipLaunchKernelGGL(task_count_kernel, dim3(grid), dim3(256), 0, c->stream, a);
+        size_t tb = temp;
+        if ((e = hipcub::DeviceScan::ExclusiveSum(base + o_tmp, tb, cnt_main, off_main, (int)n, c->stream)))
+            return hip_err(c, e, "scan");
+        tb = temp;
+        if ((e = hipcub::DeviceScan::ExclusiveSum(base + o_tmp, tb, cnt_small, off_small, (int)n, c->stream)))
+            return hip_err(c, e, "scan");
+    }
+    unsigned long long tail[2] = {0, 0};
+    unsigned int stail[2] = {0, 0}, hbad = 0;
+    if (n) {
+        if ((e = hipMemcpyAsync(tail, off_main + n - 1, 8, hipMemcpyDeviceToHost, c->stream)) ||
+            (e = hipMemcpyAsync(tail + 1, cnt_main + n - 1, 8, hipMemcpyDeviceToHost, c->stream)) ||
+            (e = hipMemcpyAsync(stail, off_small + n - 1, 4, hipMemcpyDeviceToHost, c->stream)) ||
+            (e = hipMemcpyAsync(stail + 1, cnt_small + n - 1, 4, hipMemcpyDeviceToHost, c->stream)))
+            return hip_err(c, e, "D2H task totals");
+    }
+    if (has_order && (e = hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H flag");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "task list sync");
+    if ((e = hipGetLastError())) return hip_err(c, e, "task list kernels");
+    if (hbad) return set_err(c, G2_EINVAL, "schedule is not a permutation of the sources");
+    *nmain = tail[0] + tail[1];
+    *nsmall = (uint64_t)stail[0] + stail[1];
+    if ((rc = ensure(c, c->order, std::max<uint64_t>(*nmain + *nsmall, 1) * 8, "task list"))) return rc;
+    if (n) hipLaunchKernelGGL(task_fill_kernel, dim3(grid), dim3(256), 0, c->stream, a, off_main, off_small,
+                              (uint64_t*)c->order.ptr, *nmain);
+    if ((e = hipGetLastError())) return hip_err(c, e, "task fill");
+    return G2_OK;
+}
+
 int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
             uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
@@ -1024,33 +1167,12 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     // batched small roots: roots of degree <= G go to the lane-group kernel
     const uint32_t rpw = c->tune.mce_roots_per_wave;
     const uint32_t G = account ? 0u : rpw >= 4 ? 16u : rpw >= 2 ? 32u : 0u;
-    std::vector<uint64_t> hs;  // source indices for the lane-group kernel
+    uint64_t nsmall = 0;  // roots for the lane-group kernel, after the main list
     if (G || (sched && (sched->order || sched->split || sched->end > sched->begin))) {
-        const bool ranged = sched && sched->end > sched->begin;
-        const uint64_t b = ranged ? sched->begin : 0;
-        const uint64_t en = ranged ? sched->end : c->ns;
-        std::vector<uint64_t> h;
-        h.reserve(en - b);
-        for (uint64_t t = b; t < en; t++) {
-            const uint64_t si = sched && sched->order ? sched->order[t] : t;
-            const uint64_t k = sched && sched->split ? std::max<uint32_t>(1, std::min<uint32_t>(sched->split[si], 0xffffu)) : 1;
-            if (G && k == 1 && c->h_src_deg[si] <= G) {
-                hs.push_back(si);
-                continue;
-            }
-            for (uint64_t j = 0; j < k; j++) h.push_back(si | (j << 32) | (k << 48));
-        }
-        if ((rc = ensure(c, c->order, std::max<uint64_t>(h.size() + hs.size(), 1) * 8, "task list"))) return rc;
-        if (!h.empty()) {
-            hipError_t e = hipMemcpy(c->order.ptr, h.data(), h.size() * 8, hipMemcpyHostToDevice);
-            if (e != hipSuccess) return hip_err(c, e, "H2D task list");
-        }
-        if (!hs.empty()) {
-            hipError_t e = hipMemcpy((uint64_t*)c->order.ptr + h.size(), hs.data(), hs.size() * 8, hipMemcpyHostToDevice);
-            if (e != hipSuccess) return hip_err(c, e, "H2D task list");
-        }
+        uint64_t nmain = 0;
+        if ((rc = build_tasks(c, sched, G, &nmain, &nsmall))) return rc;
         tasks = (const uint64_t*)c->order.ptr;
-        ntasks = h.size();
+        ntasks = nmain;
     }
     if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks, 1) * 8, "deferred roots"))) return rc;
     // donation queue: counters, record table and a word pool (a record is a frame plus two words)
@@ -1155,7 +1277,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     }
     float k3 = 0;
     bool k2_done = false;
-    if (!hs.empty()) {
+    if (nsmall) {
         if (nbig) {  // the large-root pass's events are reused below
             if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "mce sync");
             (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
@@ -1164,7 +1286,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         // lane-group kernel over the small roots
         a.tasks = tasks + ntasks;
         a.first = 0;
-        a.ntasks = hs.size();
+        a.ntasks = nsmall;
         a.wave_span = nullptr;
         a.donate_every = 0;
         const uint64_t gblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * (c->tune.waves_per_cu ? c->tune.waves_per_cu : 32) / bw);

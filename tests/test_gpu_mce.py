@@ -188,3 +188,30 @@ def test_roots_per_wave(g2, orc, cfg1, rpw):
         assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
         assert got["stats"]["launches"] >= 2
 
+
+
+def test_schedule_checks_and_ranges(g2, orc, gpu_exec):
+    """The device-built task list: a repeated or out-of-range source in the
+    order is rejected; schedule ranges cover exactly their positions."""
+    print("case: schedule checks and ranges", flush=True)
+    g = G.gnp(g2, 120, 0.2, 9)
+    gpu_exec.transfer(g)
+    bad = np.arange(g.nv, dtype=np.uint32)
+    bad[5] = bad[6]
+    with pytest.raises(RuntimeError, match="permutation"):
+        gpu_exec.run_mce(order=bad)
+    bad[5] = g.nv
+    with pytest.raises(RuntimeError, match="permutation"):
+        gpu_exec.run_mce(order=bad)
+    ref = orc.mce(g, per_root=True)
+    rng = np.random.default_rng(2)
+    order = rng.permutation(g.nv).astype(np.uint32)
+    split = rng.integers(1, 5, size=g.nv).astype(np.uint32)
+    tot, hist, nodes = 0, np.zeros(256, np.uint64), np.zeros(g.nv, np.uint64)
+    for b, e in ((0, 37), (37, 38), (38, g.nv)):
+        r = gpu_exec.run_mce(per_root=True, order=order, split=split, begin=b, end=e)
+        tot += r["total"]
+        hist += r["hist"]
+        nodes += r["per_root_nodes"]
+    assert tot == ref["total"] and np.array_equal(hist, ref["hist"])
+    assert np.array_equal(nodes, ref["per_root_nodes"])
