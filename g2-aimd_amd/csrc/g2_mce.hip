@@ -124,6 +124,17 @@ __device__ inline void drain() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// the k lowest set bits of w
+__device__ inline uint64_t lowest_bits(uint64_t w, uint32_t k) {
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < k && w; i++) {
+        const uint64_t b = w & (~w + 1);
+        m |= b;
+        w ^= b;
+    }
+    return m;
+}
+
 // true when idle waves outnumber the records waiting for them
 __device__ inline bool hungry(const MceArgs& a) {
     __builtin_amdgcn_wave_barrier();
@@ -336,12 +347,13 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
         bool descend = false;
         if (fresh) {
             if (base + depth > 0 || part == 0) nodes++;
-            // a task that has walked 16 x donate_every nodes is heavy: from then on,
+            // a task that has walked 64 x donate_every nodes is heavy: from then on,
             // every donate_every nodes, look for idle waves
-            if (a.donate_every && ++since >= a.donate_every && nodes >= 16ull * a.donate_every) {
+            if (a.donate_every && ++since >= a.donate_every && nodes >= 64ull * a.donate_every) {
                 since = 0;
-                // hand the shallowest frame with untried children to an idle wave
-                // (not the root frame of a split root: its children are dealt by part)
+                // hand the upper half of the shallowest frame's untried children
+                // to an idle wave (not the root frame of a split root: its
+                // children are dealt by part)
                 const uint32_t first = (base == 0 && parts > 1) ? 1u : 0u;
                 const uint64_t open = ballot(lane < depth && lane >= first && fQ != 0);
                 if (open && hungry(a)) {
@@ -350,11 +362,16 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
                     if (reserve_record(a, 6, &idx, &off)) {
                         const uint64_t fp = read_lane(fP, i), fxp = read_lane(fXP, i), fq = read_lane(fQ, i),
                                        fxx = read_lane(fXX, i);
+                        // the donor keeps the lower half; the taker's P and XP
+                        // already account for those siblings, which come first
+                        const uint32_t nq = __popcll(fq);
+                        const uint64_t keep = nq >= 2 ? lowest_bits(fq, nq - nq / 2) : 0;
                         const uint64_t w = lane == 0 ? (uint64_t)si : lane == 1 ? (uint64_t)(base + i)
-                                         : lane == 2 ? fp : lane == 3 ? fxp : lane == 4 ? fq : fxx;
+                                         : lane == 2 ? (fp & ~keep) : lane == 3 ? (fxp | keep)
+                                         : lane == 4 ? (fq & ~keep) : fxx;
                         if (lane < 6) st_dev((unsigned long long*)&a.pool[off + lane], w);
                         publish_record(a, idx, off);
-                        fQ = lane == i ? 0 : fQ;
+                        fQ = lane == i ? keep : fQ;
                     }
                 }
             }
@@ -408,7 +425,7 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
 }
 
 template <bool ACCOUNT>
-__global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
+__global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
     __shared__ unsigned long long lhist[64];
     __shared__ SmallLds small_lds[kMaxBlockWaves];
     for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x) lhist[i] = 0;
@@ -570,21 +587,40 @@ __global__ __launch_bounds__(256) void mce_kernel(MceArgs a) {
             if (fresh) {
                 if (own) root_nodes++;
                 if (!ACCOUNT && a.donate_every && ++since >= a.donate_every &&
-                    root_nodes >= 16ull * a.donate_every) {
+                    root_nodes >= 64ull * a.donate_every) {
                     since = 0;
                     if (hungry(a)) {
-                        // hand the shallowest ancestor frame with untried children to an
-                        // idle wave (not the root frame of a split root: dealt by part)
+                        // hand the upper half of the shallowest ancestor frame's untried
+                        // children to an idle wave (not the root frame of a split root)
                         for (uint64_t i = (base == 0 && parts > 1) ? 1 : 0; i < depth; i++) {
-                            const uint64_t* Fi = frames + i * FW;
-                            if (!ballot(lane < W && Fi[2 * W + lane] != 0)) continue;
+                            uint64_t* Fi = frames + i * FW;
+                            const uint64_t qword = lane < W ? Fi[2 * W + lane] : 0;
+                            if (!ballot(qword != 0)) continue;
                             uint64_t idx, off;
                             if (reserve_record(a, 2 + FW, &idx, &off)) {
-                                for (uint64_t k = lane; k < 2 + FW; k += kWave)
-                                    st_dev((unsigned long long*)&a.pool[off + k],
-                                           k == 0 ? (uint64_t)si : k == 1 ? base + i : Fi[k - 2]);
+                                // the donor keeps the lowest half of Q (over all W words)
+                                const uint32_t cnt = __popcll(qword);
+                                uint32_t incl = cnt;
+#pragma unroll
+                                for (int o = 1; o < kWave; o <<= 1) {
+                                    const uint32_t y = __shfl_up(incl, o, kWave);
+                                    if (lane >= (uint32_t)o) incl += y;
+                                }
+                                const uint32_t nq = read_lane(incl, kWave - 1), before = incl - cnt;
+                                const uint32_t keep_all = nq >= 2 ? nq - nq / 2 : 0;
+                                const uint32_t mine = keep_all > before ? min(cnt, keep_all - before) : 0u;
+                                const uint64_t keepw = lowest_bits(qword, mine);
+                                if (lane < W) {
+                                    st_dev((unsigned long long*)&a.pool[off + 2 + lane], Fi[lane] & ~keepw);
+                                    st_dev((unsigned long long*)&a.pool[off + 2 + W + lane], Fi[W + lane] | keepw);
+                                    st_dev((unsigned long long*)&a.pool[off + 2 + 2 * W + lane], qword & ~keepw);
+                                }
+                                for (uint64_t k = 3 * W + lane; k < FW; k += kWave)
+                                    st_dev((unsigned long long*)&a.pool[off + 2 + k], Fi[k]);
+                                if (lane < 2)
+                                    st_dev((unsigned long long*)&a.pool[off + lane], lane == 0 ? (uint64_t)si : base + i);
                                 publish_record(a, idx, off);
-                                if (lane < W) frames[i * FW + 2 * W + lane] = 0;
+                                if (lane < W) Fi[2 * W + lane] = keepw;
                             }
                             break;
                         }

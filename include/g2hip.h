@@ -52,9 +52,9 @@ typedef struct g2_tuning {
     uint32_t gm_chunk;            /* level-1 candidates per matching task (0 = whole root) */
     uint32_t mce_small_max;       /* MCE roots of at most this degree run in registers
                                      (0 = 64, the most; 1 = practically off) */
-    uint32_t mce_donate_every;    /* MCE: once a task has walked 16x this many nodes, the wave looks
-                                     for idle waves every this many nodes and hands them its
-                                     shallowest open frame (0 = 256; 0xffffffff = never) */
+    uint32_t mce_donate_every;    /* MCE: once a task has walked 64x this many nodes, the wave looks
+                                     for idle waves every this many nodes and hands them the upper
+                                     half of its shallowest open frame (0 = 256; 0xffffffff = never) */
     uint32_t mce_roots_per_wave;  /* MCE: 2 = roots of degree <= 32 run two per wave (lane groups
                                      of 32), 4 = degree <= 16 four per wave; 0/1 = one per wave */
 } g2_tuning;
