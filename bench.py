@@ -255,6 +255,31 @@ def run_social(args, g2, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_mce_plain(args, g2, dev):
+    """Plain maximal clique enumeration on a social-family graph of --social-n
+    vertices (optionally only roots whose degree is in --source-degree lo:hi);
+    the workload the MCE kernel profiles are taken on."""
+    S = importlib.import_module("g2-aimd_amd.sched")
+    g, _ = social_graph(g2, S, args.social_n, SOCIAL_SEED)
+    src = None
+    if args.source_degree:
+        lo, hi = (int(x) for x in args.source_degree.split(":"))
+        d = g.degrees()
+        src = np.nonzero((d >= lo) & (d <= hi))[0].astype(np.uint32)
+    with dev.Executor(device=0) as ex:
+        ex.transfer(g, src)
+        r, ms, kms, idle = time_mce(ex, args.steps, args.warmup)
+    out = {"metric": "maximal cliques/s", "value": r["total"] / (ms / 1e3), "unit": "maximal cliques/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u32 vertex ids, u64 bitsets and counts",
+           "data": "synthetic (seeded social generator)",
+           "config": {"workload": f"maximal clique enumeration on the social graph, n={args.social_n}"
+                                  + (f", roots of degree {args.source_degree}" if args.source_degree else ""),
+                      "roots": int(g.nv if src is None else len(src)), "max_cliques": r["total"],
+                      "nodes": r["stats"]["nodes"], "kernel_ms": kms, "wave_idle_frac": idle}}
+    print(json.dumps(out), flush=True)
+
+
 BATCH_SEED = 0x6732A1D0C0000004
 
 
@@ -376,22 +401,24 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MB "
                          "(rocprof counter calibration) and exit")
-    ap.add_argument("--workload", choices=("matching", "social", "batch"), default="matching",
+    ap.add_argument("--workload", choices=("matching", "social", "batch", "mce"), default="matching",
                     help="matching: 4-clique matching on R-MAT (default line); social: maximal "
                          "cliques on the social graph, plain vs learned + batched; batch: many "
-                         "random patterns, batched vs serial launches (use --scale 24)")
+                         "random patterns, batched vs serial launches (use --scale 24); mce: "
+                         "plain maximal cliques on a social graph (--social-n, --source-degree)")
     ap.add_argument("--queries", type=int, default=512)
     ap.add_argument("--tune-sample", type=int, default=32)
     ap.add_argument("--node-budget", type=int, default=1_000_000_000)
     ap.add_argument("--pilot-seconds", type=float, default=300.0)
     ap.add_argument("--social-n", type=int, default=65_000_000)
+    ap.add_argument("--source-degree", default="", help="mce workload: only roots of degree lo:hi")
     ap.add_argument("--train-n", type=int, default=2_000_000)
     ap.add_argument("--train-graphs", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
-    if args.workload in ("social", "batch") and args.gpus == 1:
+    if args.workload in ("social", "batch", "mce") and args.gpus == 1:
         g2 = importlib.import_module("g2-aimd_amd")
-        fn = run_social if args.workload == "social" else run_batch
+        fn = {"social": run_social, "batch": run_batch, "mce": run_mce_plain}[args.workload]
         fn(args, g2, importlib.import_module("g2-aimd_amd.device"))
         return
     if args.copy_calibration_mb:

@@ -679,14 +679,18 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                         }
                     };
                     const uint64_t npw = wave_sum(__popcll(pw));  // |P|
-                    if (!ACCOUNT && WX <= kWave && npw < 32) {
+                    // column scoring costs ~|P|/4 rounds of loads, candidate scoring
+                    // ~(nonzero X words)/kScoreWords rounds: take the cheaper
+                    const uint64_t xw0 = (WX <= kWave && lane < WX) ? XX[lane] : 0;
+                    const uint64_t nzx = __popcll(ballot(xw0 != 0));
+                    if (!ACCOUNT && WX <= kWave && npw < 32 && (npw + 3) / 4 < (nzx + kScoreWords - 1) / kScoreWords) {
                         // Few P members: score all X candidates at once from the
                         // columns of adjX (score(x) = sum over j in P of bit x of
                         // adjX[j], the same number as popc(adjP[x] & P) since the
                         // bitsets are symmetric).  Lane i adds word i of each
                         // member's row into 5 bit-sliced counters, then keeps the
                         // candidates of its word with the largest count.
-                        const uint64_t xw = lane < WX ? XX[lane] : 0;
+                        const uint64_t xw = xw0;
                         uint64_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
                         for (uint32_t wi = 0; wi < W; wi++) {
                             uint64_t word = read_lane(pw, wi);
