@@ -128,7 +128,7 @@ void g2_destroy(g2_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
                       &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans,
-                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate, &c->taskbuild})
+                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate, &c->taskbuild, &c->orient})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);

@@ -6,27 +6,41 @@
 // root.  Then each match lies in U(v0) = { w in N(v0) : w above v0 } and the
 // plan's back-neighbour / condition tables become bit operations over U.
 //
-// Orientation: "above" compares the key (degree, id) instead of the id.  The
-// plan's condition pairs are a complete symmetry breaking (checked for every
-// ranking of the query vertices when the plan is set), so for any total order
-// of the data vertices exactly one member of each automorphism class of an
-// embedding satisfies them: the total count over all roots is the same number
-// the reference's id order gives (app_gmatch/gmatch.h:559-588), while |U| is
-// bounded by O(sqrt(E)) instead of by the largest degree.
+// Orientation: "above" compares the rank pi(v) of the key (degree, id)
+// instead of the id.  The plan's condition pairs are a complete symmetry
+// breaking (checked for every ranking of the query vertices when the plan is
+// set), so for any total order of the data vertices exactly one member of
+// each automorphism class of an embedding satisfies them: the total count
+// over all roots is the same number the reference's id order gives
+// (app_gmatch/gmatch.h:559-588), while |U| is bounded by O(sqrt(E)) instead
+// of by the largest degree.
 //
-// Per root (one wave64 each, dynamic fetch):
-//   1. U = ballot-compacted { w in N(v0) : key(w) > key(v0) }, id order;
-//   2. rank of each member inside U by key (local bit index = rank), so the
-//      order conditions become contiguous bit ranges;
-//   3. rows[r] = N(U_r) & U as m-bit sets, one wave pass per member (stream
-//      N(u) against U, or look U up in N(u) when that row is long);
-//   4. each lane walks the matches of one position-1 candidate depth first:
+// Each run first builds the oriented relabelled CSR on the device:
+//   pi      = rank of (degree, id) over all vertices (radix sort),
+//   N+(x)   = { pi(w) : w in N(v), pi(w) > x } for x = pi(v), sorted.
+// U(x) is then the row N+(x) itself, already in rank order, and the rows of
+// the local graph come from the short oriented rows N+(u) of its members
+// instead of their full rows (a hub sits high in the order, so its N+ is
+// short while its full row is the longest).
+//
+// Per root x (one wave64 each, dynamic fetch):
+//   1. U = N+(x), m = |U|, bit index = position in U;
+//   2. rows[i] = N(U_i) & U as m-bit sets, from N+(U_i) & U[i+1..m) (both
+//      bits set per found pair), by one flattened work list per 64 members
+//      (stream N+(U_i) against U, or look U up in N+(U_i) when that row is
+//      long);
+//   3. each lane walks the matches of one position-1 candidate depth first:
 //      cand(i) = AND of the rows of its back neighbours & order ranges &
-//      NON_EQUAL holes; the last position is a popcount.
-// Roots whose arena need exceeds the per-wave size are deferred to a second
-// launch with larger arenas.
+//      NON_EQUAL holes, recomputed word by word (no stacks); the last
+//      position is a popcount over the words its order range leaves.
+// Rows of roots with m <= 256 and m * ceil(m/64) <= 448 words live in LDS
+// (4.5 KB per wave, 8 blocks of 4 waves per CU); larger roots use a global
+// per-wave arena, and roots beyond that are deferred to a second launch
+// with larger arenas.
 #include <algorithm>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "g2_device.hpp"
 #include "g2_internal.hpp"
@@ -35,10 +49,13 @@ namespace g2 {
 
 enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_NBIG, LCTR_MAXNEED, LCTR_ERR, LCTR_N };
 
+constexpr int kLocalBlockWaves = 4;     // __launch_bounds__(256)
+constexpr uint32_t kLRowWords = 448;    // per-wave LDS rows (u64 words)
+constexpr uint32_t kLU = 256;           // per-wave LDS copy of U (u32)
+
 struct LocalArgs {
-    const uint64_t* __restrict__ rp;
-    const uint32_t* __restrict__ cols;
-    const uint32_t* __restrict__ deg;
+    const uint64_t* __restrict__ orp;   // oriented rows over ranks
+    const uint32_t* __restrict__ ocols;
     const uint32_t* __restrict__ list;  // roots of the large-root pass (nullable: roots = [0, ntasks))
     uint64_t ntasks;
     LocalProgram prog;
@@ -48,13 +65,8 @@ struct LocalArgs {
     unsigned long long* ctr;
 };
 
-// u64 words of a root's arena: ids (m u32) + keys (m) + ranks (m u32) +
-// rows (m*W) + per-lane DFS stacks (64 lanes * (n-2) levels * W)
-__host__ __device__ inline uint64_t local_need(uint64_t m, uint32_t n) {
-    const uint64_t W = (m + 63) / 64;
-    const uint64_t lv = n > 2 ? n - 2 : 0;
-    return (m + 1) / 2 + m + (m + 1) / 2 + m * W + 64 * lv * W + 8;
-}
+// u64 words of a root's global arena: the local rows
+__host__ __device__ inline uint64_t local_need(uint64_t m) { return m * ((m + 63) / 64); }
 
 __device__ inline uint64_t bits_above(uint64_t r, uint64_t w) {  // bits > r within word w
     const uint64_t lo = w * 64;
@@ -70,9 +82,9 @@ __device__ inline uint64_t bits_below(uint64_t r, uint64_t w) {  // bits < r wit
     return (1ull << (r - lo)) - 1;
 }
 
-// candidates of position i as a W-word set given the matched ranks M[1..i-1]
+// word w of the candidates of position i given the matched ranks M[1..i-1]
 __device__ inline uint64_t cand_word(const LocalProgram& P, uint32_t i, const uint32_t* M,
-                                     const uint64_t* rows, uint64_t W, uint64_t m, uint64_t w) {
+                                     const uint64_t* rows, uint32_t W, uint32_t m, uint32_t w) {
     uint64_t x = (w + 1) * 64 <= m ? ~0ull : ((1ull << (m - w * 64)) - 1);
     uint32_t b = P.bn_mask[i];
     while (b) {
@@ -101,162 +113,192 @@ __device__ inline uint64_t cand_word(const LocalProgram& P, uint32_t i, const ui
     return x;
 }
 
+// candidate positions of position i lie in [*lo, *hi) by its order conditions
+__device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint32_t* M, uint32_t m,
+                                  uint32_t* lo, uint32_t* hi) {
+    uint32_t a = 0, b = m;
+    uint32_t g = P.gt_mask[i];
+    while (g) {
+        const uint32_t j = __builtin_ctz(g);
+        g &= g - 1;
+        a = max(a, M[j] + 1);
+    }
+    uint32_t l = P.lt_mask[i];
+    while (l) {
+        const uint32_t j = __builtin_ctz(l);
+        l &= l - 1;
+        b = min(b, M[j]);
+    }
+    *lo = a;
+    *hi = b;
+}
+
+// One root: build the local rows over U, then count.  rows and U point into
+// LDS or into global memory (the call sites fix which after inlining).
+template <int N>
+__device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint32_t m, uint64_t* rows,
+                                           const uint32_t* U, uint32_t* Ucopy, uint64_t& lane_total,
+                                           uint64_t& nodes) {
+    const uint32_t lane = lane_id();
+    const uint32_t W = (m + 63) / 64;
+    const LocalProgram& P = a.prog;
+    for (uint32_t i = lane; i < m * W; i += kWave) rows[i] = 0;
+    if (Ucopy)
+        for (uint32_t i = lane; i < m; i += kWave) Ucopy[i] = a.ocols[lo + i];
+    wave_sync();
+
+    // ---- local rows: pairs (i, j), i < j, with U_j in N+(U_i) -------------
+    for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) {
+        const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
+        uint64_t st = 0;
+        uint32_t code = 0, len = 0;
+        if (lane < na) {
+            const uint32_t ia = a0 + lane, u = U[ia];
+            uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
+            if (e1 - e0 > kWave) {  // restrict a long row to [U[ia+1], U[m-1]]
+                e0 = lower_bound(a.ocols, e0, e1, U[ia + 1]);
+                e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
+            }
+            const uint64_t n = e1 - e0, rem = m - 1 - ia;
+            const bool rev = rem * (1 + bitlen(n)) < n * (1 + bitlen(rem));
+            st = e0;
+            code = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
+            len = rev ? (uint32_t)rem : (uint32_t)n;
+        }
+        uint32_t incl = len;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, kWave);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        const uint32_t pref = incl - len;
+        const uint32_t T = read_lane(incl, kWave - 1);
+        const uint32_t st_lo = (uint32_t)st, st_hi = (uint32_t)(st >> 32);
+        for (uint32_t f0 = 0; f0 < T; f0 += kWave) {
+            const uint32_t f = f0 + lane;
+            // owner: the last member whose items start at or before f
+            uint32_t j = 0;
+#pragma unroll
+            for (uint32_t step = 32; step >= 1; step >>= 1) {
+                const uint32_t c = j + step;
+                const uint32_t pc = __shfl(pref, (int)c, kWave);
+                if (c < na && pc <= f) j = c;
+            }
+            const uint32_t item = f - __shfl(pref, (int)j, kWave);
+            const uint32_t cj = __shfl(code, (int)j, kWave);
+            const uint64_t sj = ((uint64_t)__shfl(st_hi, (int)j, kWave) << 32) | __shfl(st_lo, (int)j, kWave);
+            if (f < T) {
+                const uint32_t ia = a0 + j;
+                uint32_t ib;
+                bool hit;
+                if (cj & 0x80000000u) {
+                    ib = ia + 1 + item;
+                    hit = contains(a.ocols, sj, sj + (cj & 0x7fffffffu), U[ib]);
+                } else {
+                    const uint32_t w = a.ocols[sj + item];
+                    ib = (uint32_t)lower_bound(U, ia + 1, m, w);
+                    hit = ib < m && U[ib] == w;
+                }
+                if (hit) {
+                    atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+                    atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                }
+            }
+        }
+    }
+    wave_sync();
+
+    // ---- per-lane depth-first walk from each position-1 candidate ---------
+    uint32_t M[N];
+    uint32_t cur[N], hiv[N];
+    for (uint32_t r1 = lane; r1 < m; r1 += kWave) {
+        M[1] = r1;
+        uint32_t plo, phi;
+        if (N == 3) {
+            cand_range(P, 2, M, m, &plo, &phi);
+            if (plo < phi)
+                for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, 2, M, rows, W, m, w));
+            nodes++;
+            continue;
+        }
+        nodes++;
+        uint32_t level = 2;
+        cand_range(P, 2, M, m, &cur[2], &hiv[2]);
+        while (true) {
+            // next member of cand(level) at or after cur[level]
+            uint32_t pos = cur[level];
+            const uint32_t hi = hiv[level];
+            bool found = false;
+            while (pos < hi) {
+                const uint32_t w = pos >> 6;
+                const uint64_t x = cand_word(P, level, M, rows, W, m, w) & (~0ull << (pos & 63));
+                if (x) {
+                    pos = w * 64 + __builtin_ctzll(x);
+                    found = pos < hi;
+                    break;
+                }
+                pos = (w + 1) * 64;
+            }
+            if (!found) {
+                if (level == 2) break;
+                level--;
+                continue;
+            }
+            cur[level] = pos + 1;
+            M[level] = pos;
+            const uint32_t nx = level + 1;
+            nodes++;
+            cand_range(P, nx, M, m, &plo, &phi);
+            if (nx == N - 1) {
+                if (plo < phi)
+                    for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++)
+                        lane_total += __popcll(cand_word(P, nx, M, rows, W, m, w));
+            } else {
+                cur[nx] = plo;
+                hiv[nx] = phi;
+                level = nx;
+            }
+        }
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(256) void gm_local_kernel(LocalArgs a) {
+    __shared__ uint64_t lrows[kLocalBlockWaves][kLRowWords];
+    __shared__ uint32_t lU[kLocalBlockWaves][kLU];
     const uint32_t lane = lane_id();
-    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wid;
     uint64_t* ws = a.scratch + gwave * a.scratch_words;
     uint64_t lane_total = 0, nodes = 0;
-    const LocalProgram& P = a.prog;
 
     while (true) {
         const uint64_t t = fetch_task(&a.ctr[LCTR_NEXT]);
         if (t >= a.ntasks) break;
-        const uint32_t v0 = uniform(a.list ? a.list[t] : (uint32_t)t);
-        const uint64_t lo = uniform(a.rp[v0]), hi = uniform(a.rp[v0 + 1]);
-        if (hi - lo < (uint64_t)(N - 1)) continue;
-        const uint64_t key0 = ((uint64_t)a.deg[v0] << 32) | v0;
-
-        // 1. U in id order; the arena is only touched once the need is known,
-        //    so count first
-        uint64_t m = 0;
-        for (uint64_t base = lo; base < hi; base += kWave) {
-            const uint64_t e = base + lane;
-            bool up = false;
-            if (e < hi) {
-                const uint32_t w = a.cols[e];
-                up = (((uint64_t)a.deg[w] << 32) | w) > key0;
+        const uint32_t x = uniform(a.list ? a.list[t] : (uint32_t)t);
+        const uint64_t lo = uniform(a.orp[x]);
+        const uint32_t m = (uint32_t)(uniform(a.orp[x + 1]) - lo);
+        if (m < (uint32_t)(N - 1)) continue;
+        if constexpr (N == 2) {
+            if (lane == 0) lane_total += m;
+            continue;
+        } else {
+            const uint64_t need = local_need(m);
+            if (m <= kLU && need <= kLRowWords) {
+                local_root<N>(a, lo, m, lrows[wid], lU[wid], lU[wid], lane_total, nodes);
+                continue;
             }
-            m += __popcll(ballot(up));
-        }
-        if (m < (uint64_t)(N - 1)) continue;
-        const uint64_t W = (m + 63) / 64;
-        const uint64_t need = local_need(m, N);
-        if (need > a.scratch_words) {
+            if (need <= a.scratch_words) {
+                local_root<N>(a, lo, m, ws, a.ocols + lo, nullptr, lane_total, nodes);
+                continue;
+            }
             if (lane == 0) {
                 if (a.big_list) {
                     const unsigned long long idx = atomicAdd(&a.ctr[LCTR_NBIG], 1ull);
-                    a.big_list[idx] = v0;
+                    a.big_list[idx] = x;
                     atomicMax(&a.ctr[LCTR_MAXNEED], (unsigned long long)need);
                 } else {
                     atomicMax(&a.ctr[LCTR_ERR], (unsigned long long)need);
-                }
-            }
-            continue;
-        }
-        uint32_t* Uid = (uint32_t*)ws;
-        uint64_t* Ukey = ws + (m + 1) / 2;
-        uint32_t* Urank = (uint32_t*)(Ukey + m);
-        uint64_t* rows = Ukey + m + (m + 1) / 2;
-        uint64_t* stacks = rows + m * W;
-        uint64_t k = 0;
-        for (uint64_t base = lo; base < hi; base += kWave) {
-            const uint64_t e = base + lane;
-            bool up = false;
-            uint32_t w = 0;
-            uint64_t kw = 0;
-            if (e < hi) {
-                w = a.cols[e];
-                kw = ((uint64_t)a.deg[w] << 32) | w;
-                up = kw > key0;
-            }
-            const uint64_t bm = ballot(up);
-            if (up) {
-                const uint64_t p = k + lanes_below(bm);
-                Uid[p] = w;
-                Ukey[p] = kw;
-            }
-            k += __popcll(bm);
-        }
-        for (uint64_t i = lane; i < m * W; i += kWave) rows[i] = 0;
-        wave_sync();
-        // 2. rank of every member by key
-        for (uint64_t j = lane; j < m; j += kWave) {
-            const uint64_t kj = Ukey[j];
-            uint32_t r = 0;
-            for (uint64_t i = 0; i < m; i++) r += Ukey[i] < kj;
-            Urank[j] = r;
-        }
-        wave_sync();
-        // 3. rows over ranks
-        const uint32_t umin = Uid[0], umax = Uid[m - 1];
-        for (uint64_t j = 0; j < m; j++) {
-            const uint32_t u = Uid[j];
-            const uint64_t rj = Urank[j];
-            const uint64_t ulo = a.rp[u], uhi = a.rp[u + 1];
-            const uint64_t s0 = lower_bound(a.cols, ulo, uhi, umin);
-            const uint64_t s1 = lower_bound(a.cols, s0, uhi, umax + 1);
-            uint64_t* row = rows + rj * W;
-            if (s1 - s0 <= m * (1 + bitlen(m))) {
-                for (uint64_t e = s0 + lane; e < s1; e += kWave) {
-                    const uint32_t w = a.cols[e];
-                    const uint64_t p = lower_bound(Uid, 0, m, w);
-                    if (p < m && Uid[p] == w) {
-                        const uint32_t rp_ = Urank[p];
-                        atomicOr((unsigned long long*)&row[rp_ >> 6], 1ull << (rp_ & 63));
-                    }
-                }
-            } else {
-                for (uint64_t p = lane; p < m; p += kWave) {
-                    if (contains(a.cols, s0, s1, Uid[p])) {
-                        const uint32_t rp_ = Urank[p];
-                        atomicOr((unsigned long long*)&row[rp_ >> 6], 1ull << (rp_ & 63));
-                    }
-                }
-            }
-        }
-        wave_sync();
-        // 4. per-lane depth-first walk from each position-1 candidate
-        if (N == 2) {
-            if (lane == 0) lane_total += m;
-            continue;
-        }
-        uint64_t* stk = stacks + (uint64_t)lane * (N - 2) * W;  // level i at stk[(i-2)*W]
-        uint32_t M[N];
-        uint64_t cur[N];
-        for (uint64_t r1 = lane; r1 < m; r1 += kWave) {
-            M[1] = (uint32_t)r1;
-            // level 2
-            if (N == 3) {
-                for (uint64_t w = 0; w < W; w++) lane_total += __popcll(cand_word(P, 2, M, rows, W, m, w));
-                nodes++;
-                continue;
-            }
-            for (uint64_t w = 0; w < W; w++) stk[w] = cand_word(P, 2, M, rows, W, m, w);
-            nodes++;
-            uint32_t level = 2;
-            cur[2] = 0;
-            while (true) {
-                uint64_t* cs = stk + (uint64_t)(level - 2) * W;
-                // next member of cand(level) at or after cur[level]
-                uint64_t pos = cur[level];
-                bool found = false;
-                while (pos < m) {
-                    const uint64_t w = pos >> 6;
-                    const uint64_t x = cs[w] & (~0ull << (pos & 63));
-                    if (x) {
-                        pos = w * 64 + __builtin_ctzll(x);
-                        found = true;
-                        break;
-                    }
-                    pos = (w + 1) * 64;
-                }
-                if (!found) {
-                    if (level == 2) break;
-                    level--;
-                    continue;
-                }
-                cur[level] = pos + 1;
-                M[level] = (uint32_t)pos;
-                const uint32_t nx = level + 1;
-                nodes++;
-                if (nx == N - 1) {
-                    for (uint64_t w = 0; w < W; w++) lane_total += __popcll(cand_word(P, nx, M, rows, W, m, w));
-                } else {
-                    uint64_t* ns = stk + (uint64_t)(nx - 2) * W;
-                    for (uint64_t w = 0; w < W; w++) ns[w] = cand_word(P, nx, M, rows, W, m, w);
-                    cur[nx] = 0;
-                    level = nx;
                 }
             }
         }
@@ -267,6 +309,131 @@ __global__ __launch_bounds__(256) void gm_local_kernel(LocalArgs a) {
         if (tot) atomicAdd(&a.ctr[LCTR_COUNT], (unsigned long long)tot);
         if (nd) atomicAdd(&a.ctr[LCTR_NODES], (unsigned long long)nd);
     }
+}
+
+// ---- oriented relabelled CSR ------------------------------------------------
+
+__global__ __launch_bounds__(256) void okey_kernel(const uint32_t* __restrict__ deg, uint64_t nv,
+                                                   unsigned long long* __restrict__ keys) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (uint64_t)gridDim.x * blockDim.x)
+        keys[i] = ((unsigned long long)deg[i] << 32) | i;
+}
+
+__global__ __launch_bounds__(256) void opi_kernel(const unsigned long long* __restrict__ sorted, uint64_t nv,
+                                                  uint32_t* __restrict__ pi) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (uint64_t)gridDim.x * blockDim.x)
+        pi[(uint32_t)sorted[i]] = (uint32_t)i;
+}
+
+// out-degree of every rank: one wave per vertex
+__global__ __launch_bounds__(256) void ocount_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols,
+                                                     const uint32_t* __restrict__ pi, uint64_t nv,
+                                                     unsigned long long* __restrict__ cnt) {
+    const uint32_t lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < nv; v += nw) {
+        const uint32_t x = pi[v];
+        const uint64_t e1 = rp[v + 1];
+        uint64_t k = 0;
+        for (uint64_t e = rp[v] + lane; e < e1; e += kWave) k += pi[cols[e]] > x;
+        k = wave_sum(k);
+        if (lane == 0) cnt[x] = k;
+    }
+}
+
+// N+(x), sorted: compact the above-neighbours' ranks into tmp, then place each
+// at its rank within the row (ties, which a simple graph has none of, by slot)
+__global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols,
+                                                    const uint32_t* __restrict__ pi, uint64_t nv,
+                                                    const unsigned long long* __restrict__ orp,
+                                                    uint32_t* __restrict__ tmp, uint32_t* __restrict__ ocols) {
+    const uint32_t lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < nv; v += nw) {
+        const uint32_t x = uniform(pi[v]);
+        const uint64_t base = uniform((uint64_t)orp[x]);
+        const uint64_t m = uniform((uint64_t)orp[x + 1]) - base;
+        if (m == 0) continue;
+        const uint64_t e1 = rp[v + 1];
+        uint64_t k = 0;
+        for (uint64_t e0 = rp[v]; e0 < e1; e0 += kWave) {
+            const uint64_t e = e0 + lane;
+            uint32_t y = 0;
+            bool up = false;
+            if (e < e1) {
+                y = pi[cols[e]];
+                up = y > x;
+            }
+            const uint64_t bm = ballot(up);
+            if (up) tmp[base + k + lanes_below(bm)] = y;
+            k += __popcll(bm);
+        }
+        wave_sync();
+        for (uint64_t t0 = 0; t0 < m; t0 += kWave) {
+            const uint64_t i = t0 + lane;
+            const uint32_t my = i < m ? tmp[base + i] : 0xffffffffu;
+            uint64_t r = 0;
+            for (uint64_t c0 = 0; c0 < m; c0 += kWave) {
+                const uint32_t vc = c0 + lane < m ? tmp[base + c0 + lane] : 0xffffffffu;
+                const uint32_t nc = (uint32_t)min<uint64_t>(kWave, m - c0);
+                for (uint32_t q = 0; q < nc; q++) {
+                    const uint32_t o = read_lane(vc, q);
+                    r += o < my || (o == my && c0 + q < i);
+                }
+            }
+            if (i < m) ocols[base + r] = my;
+        }
+    }
+}
+
+// Build pi, orp and ocols of the uploaded view on the context's stream.
+static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocols) {
+    const uint64_t nv = c->nv, ne = c->ne;
+    // keys (degree << 32 | id): the degree needs bitlen(max degree) bits above the id
+    const int nbits = (int)(32 + std::max<uint64_t>(1, 64 - __builtin_clzll(std::max<uint64_t>(c->max_degree, 1))));
+    size_t t_sort = 0, t_scan = 0;
+    hipError_t e;
+    if ((e = hipcub::DeviceRadixSort::SortKeys(nullptr, t_sort, (unsigned long long*)nullptr,
+                                                (unsigned long long*)nullptr, (int)nv, 0, nbits, c->stream)))
+        return hip_err(c, e, "sort size");
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, (unsigned long long*)nullptr,
+                                               (unsigned long long*)nullptr, (int)(nv + 1), c->stream)))
+        return hip_err(c, e, "scan size");
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    // layout: keys u64[nv] | sorted u64[nv] | cnt u64[nv+1] | orp u64[nv+1] | pi u32[nv] |
+    //         tmp u32[ne] | ocols u32[ne] | temp
+    const uint64_t o_sorted = al(nv * 8), o_cnt = o_sorted + al(nv * 8), o_orp = o_cnt + al((nv + 1) * 8),
+                   o_pi = o_orp + al((nv + 1) * 8), o_tmp = o_pi + al(nv * 4), o_ocols = o_tmp + al(ne * 4),
+                   o_temp = o_ocols + al(std::max<uint64_t>(ne, 1) * 4), total = o_temp + al(std::max(t_sort, t_scan));
+    int rc;
+    if ((rc = ensure(c, c->orient, total, "oriented rows"))) return rc;
+    char* base = (char*)c->orient.ptr;
+    auto* keys = (unsigned long long*)base;
+    auto* sorted = (unsigned long long*)(base + o_sorted);
+    auto* cnt = (unsigned long long*)(base + o_cnt);
+    auto* d_orp = (unsigned long long*)(base + o_orp);
+    auto* pi = (uint32_t*)(base + o_pi);
+    auto* tmp = (uint32_t*)(base + o_tmp);
+    auto* d_ocols = (uint32_t*)(base + o_ocols);
+    const auto* rp = (const uint64_t*)c->rp.ptr;
+    const auto* cols = (const uint32_t*)c->cols.ptr;
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((nv + 255) / 256, 8192));
+    const uint64_t wgrid = (uint64_t)c->num_cus * 16 / 4;
+    hipLaunchKernelGGL(okey_kernel, dim3(grid), dim3(256), 0, c->stream, (const uint32_t*)c->deg.ptr, nv, keys);
+    size_t tb = t_sort;
+    if ((e = hipcub::DeviceRadixSort::SortKeys(base + o_temp, tb, keys, sorted, (int)nv, 0, nbits, c->stream)))
+        return hip_err(c, e, "rank sort");
+    hipLaunchKernelGGL(opi_kernel, dim3(grid), dim3(256), 0, c->stream, sorted, nv, pi);
+    if ((e = hipMemsetAsync(cnt + nv, 0, 8, c->stream))) return hip_err(c, e, "memset count");
+    hipLaunchKernelGGL(ocount_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, pi, nv, cnt);
+    tb = t_scan;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(base + o_temp, tb, cnt, d_orp, (int)(nv + 1), c->stream)))
+        return hip_err(c, e, "oriented scan");
+    hipLaunchKernelGGL(ofill_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, pi, nv, d_orp, tmp, d_ocols);
+    if ((e = hipGetLastError())) return hip_err(c, e, "oriented rows kernels");
+    *orp = (const uint64_t*)d_orp;
+    *ocols = d_ocols;
+    return G2_OK;
 }
 
 // ---- plan analysis (host) ---------------------------------------------------
@@ -347,7 +514,8 @@ static void launch_local(uint64_t blocks, uint32_t bw, hipStream_t s, const Loca
 }
 
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
-    const uint32_t bw = c->tune.block_waves;
+    if (c->nv + 1 > 0x7fffffffull) return set_err(c, G2_EINVAL, "local-bitmap matching supports fewer than 2^31-1 vertices");
+    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kLocalBlockWaves);
     const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM_LOCAL) / bw);
     const uint64_t nwaves = nblocks * bw;
     const uint64_t sw = c->tune.wave_scratch_words;
@@ -357,11 +525,15 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if ((rc = ensure(c, c->big_list2, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
+    float k1 = 0, k2 = 0, tot = 0;
+    if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
+    const uint64_t* orp = nullptr;
+    const uint32_t* ocols = nullptr;
+    if ((rc = build_oriented(c, &orp, &ocols))) return rc;
     if ((e = hipMemsetAsync(ctr, 0, LCTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
     LocalArgs a{};
-    a.rp = (const uint64_t*)c->rp.ptr;
-    a.cols = (const uint32_t*)c->cols.ptr;
-    a.deg = (const uint32_t*)c->deg.ptr;
+    a.orp = orp;
+    a.ocols = ocols;
     a.list = nullptr;
     a.ntasks = c->nv;
     a.prog = c->lprog;
@@ -381,8 +553,6 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         }
         return hipGetLastError();
     };
-    float k1 = 0, k2 = 0, tot = 0;
-    if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
     if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
     if ((e = launch(nblocks))) return hip_err(c, e, "local launch");
     if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");

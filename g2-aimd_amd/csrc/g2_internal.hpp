@@ -89,6 +89,7 @@ struct g2_ctx {
     bool local_ok = false;        // the plan qualifies for the oriented local-bitmap path
     g2::LocalProgram lprog{};
     g2::DevBuf big_list2;
+    g2::DevBuf orient;            // oriented relabelled rows of the local-bitmap path
     g2_plan plan{};
 
     // batched matching

@@ -111,13 +111,26 @@ def test_local_path_cliques(g2, orc, gpu_exec):
 
 
 def test_local_path_large_root_pass(g2, orc):
+    """Roots whose local rows exceed the LDS (|U| > 149 here) take the global
+    per-wave arena; with tiny arenas they are deferred to the second launch."""
     print("case: local path large roots", flush=True)
-    g = g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
-    with g2.Executor(device=0, wave_scratch_words=200, big_waves=32) as ex:
-        ex.transfer(g)
-        for q in (0, 24, 25):
-            p = g2.plan_for_preset(q)
-            ex.set_plan(p)
-            got = ex.run_gm()
-            assert got["stats"]["big_roots"] > 0
-            assert got["count"] == orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
+    g = G.gnp(g2, 400, 0.5, 7)
+    for tuning, deferred in (({}, False), ({"wave_scratch_words": 200, "big_waves": 32}, True)):
+        with g2.Executor(device=0, **tuning) as ex:
+            ex.transfer(g)
+            for q in (0, 24):
+                p = g2.plan_for_preset(q)
+                ex.set_plan(p)
+                got = ex.run_gm()
+                assert (got["stats"]["big_roots"] > 0) == deferred
+                assert got["count"] == orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
+
+
+def test_local_path_complete_graph(g2, gpu_exec):
+    """K_170: every root's U is the rest of the order (|U| up to 169), and the
+    k-clique totals are the binomials C(170, k)."""
+    print("case: local path complete graph", flush=True)
+    gpu_exec.transfer(G.complete(g2, 170))
+    for q, k in ((0, 3), (24, 4)):
+        gpu_exec.set_plan(g2.plan_for_preset(q))
+        assert gpu_exec.run_gm()["count"] == math.comb(170, k)
