@@ -47,11 +47,18 @@
 
 namespace g2 {
 
-enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_NBIG, LCTR_MAXNEED, LCTR_ERR, LCTR_N };
+// counters: next task, matches, nodes, deferred roots of pass 1 and their
+// largest need, the same for pass 2, largest need that fit no arena
+enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_NB1, LCTR_NEED1, LCTR_NB2, LCTR_NEED2, LCTR_ERR, LCTR_N };
 
-constexpr int kLocalBlockWaves = 4;     // __launch_bounds__(256)
+constexpr int kLocalBlockWaves = 4;     // gm_local_kernel: __launch_bounds__(256)
 constexpr uint32_t kLRowWords = 448;    // per-wave LDS rows (u64 words)
 constexpr uint32_t kLU = 256;           // per-wave LDS copy of U (u32)
+constexpr int kBWaves = 16;             // gm_local_block_kernel: __launch_bounds__(1024)
+constexpr uint32_t kBRowWords = 17664;  // per-workgroup LDS rows (138 KiB): |U| = 1039 at stride 17
+constexpr uint32_t kBU = 1040;          // per-workgroup |U| limit (at most blockDim.x + 64)
+constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
+constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
 
 struct LocalArgs {
     const uint64_t* __restrict__ orp;   // oriented rows over ranks
@@ -61,12 +68,17 @@ struct LocalArgs {
     LocalProgram prog;
     uint64_t* scratch;
     uint64_t scratch_words;
-    uint32_t* big_list;                 // deferred roots (null in the large-root pass)
+    uint32_t* big_list;                 // deferred roots (null in the last pass)
+    uint32_t defer_ctr;                 // LCTR_NB1 or LCTR_NB2: count of big_list, need next to it
     unsigned long long* ctr;
 };
 
-// u64 words of a root's global arena: the local rows
-__host__ __device__ inline uint64_t local_need(uint64_t m) { return m * ((m + 63) / 64); }
+// u64 words between consecutive local rows: ceil(m/64) rounded up to odd, so
+// that lanes reading the rows of different ranks spread over the LDS banks
+__host__ __device__ inline uint32_t row_stride(uint64_t m) { return (uint32_t)((m + 63) / 64) | 1u; }
+
+// u64 words of a root's local rows
+__host__ __device__ inline uint64_t local_need(uint64_t m) { return m * row_stride(m); }
 
 __device__ inline uint64_t bits_above(uint64_t r, uint64_t w) {  // bits > r within word w
     const uint64_t lo = w * 64;
@@ -133,137 +145,166 @@ __device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint3
     *hi = b;
 }
 
-// One root: build the local rows over U, then count.  rows and U point into
-// LDS or into global memory (the call sites fix which after inlining).
-template <int N>
-__device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint32_t m, uint64_t* rows,
-                                           const uint32_t* U, uint32_t* Ucopy, uint64_t& lane_total,
-                                           uint64_t& nodes) {
+// Local rows of members a0 .. a0+63: pairs (i, j), i < j, with U_j in
+// N+(U_i), both bits set.  One flattened work list keeps all lanes busy:
+// an item is one entry of N+(U_i) (looked up in U by binary search) or one
+// later member U_j (looked up in N+(U_i)), whichever is cheaper for i.
+// rows and U are LDS or global memory (fixed per call site after inlining).
+__device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* U, uint64_t* rows, uint32_t m,
+                                            uint32_t W, uint32_t a0) {
     const uint32_t lane = lane_id();
-    const uint32_t W = (m + 63) / 64;
-    const LocalProgram& P = a.prog;
-    for (uint32_t i = lane; i < m * W; i += kWave) rows[i] = 0;
-    if (Ucopy)
-        for (uint32_t i = lane; i < m; i += kWave) Ucopy[i] = a.ocols[lo + i];
-    wave_sync();
-
-    // ---- local rows: pairs (i, j), i < j, with U_j in N+(U_i) -------------
-    for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) {
-        const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
-        uint64_t st = 0;
-        uint32_t code = 0, len = 0;
-        if (lane < na) {
-            const uint32_t ia = a0 + lane, u = U[ia];
-            uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
-            if (e1 - e0 > kWave) {  // restrict a long row to [U[ia+1], U[m-1]]
-                e0 = lower_bound(a.ocols, e0, e1, U[ia + 1]);
-                e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
-            }
-            const uint64_t n = e1 - e0, rem = m - 1 - ia;
-            const bool rev = rem * (1 + bitlen(n)) < n * (1 + bitlen(rem));
-            st = e0;
-            code = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
-            len = rev ? (uint32_t)rem : (uint32_t)n;
+    const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
+    uint64_t st = 0;
+    uint32_t code = 0, len = 0;
+    if (lane < na) {
+        const uint32_t ia = a0 + lane, u = U[ia];
+        uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
+        if (e1 - e0 > kWave) {  // restrict a long row to [U[ia+1], U[m-1]]
+            e0 = lower_bound(a.ocols, e0, e1, U[ia + 1]);
+            e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
         }
-        uint32_t incl = len;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, kWave);
-            if (lane >= (uint32_t)o) incl += y;
-        }
-        const uint32_t pref = incl - len;
-        const uint32_t T = read_lane(incl, kWave - 1);
-        const uint32_t st_lo = (uint32_t)st, st_hi = (uint32_t)(st >> 32);
-        for (uint32_t f0 = 0; f0 < T; f0 += kWave) {
-            const uint32_t f = f0 + lane;
-            // owner: the last member whose items start at or before f
-            uint32_t j = 0;
-#pragma unroll
-            for (uint32_t step = 32; step >= 1; step >>= 1) {
-                const uint32_t c = j + step;
-                const uint32_t pc = __shfl(pref, (int)c, kWave);
-                if (c < na && pc <= f) j = c;
-            }
-            const uint32_t item = f - __shfl(pref, (int)j, kWave);
-            const uint32_t cj = __shfl(code, (int)j, kWave);
-            const uint64_t sj = ((uint64_t)__shfl(st_hi, (int)j, kWave) << 32) | __shfl(st_lo, (int)j, kWave);
-            if (f < T) {
-                const uint32_t ia = a0 + j;
-                uint32_t ib;
-                bool hit;
-                if (cj & 0x80000000u) {
-                    ib = ia + 1 + item;
-                    hit = contains(a.ocols, sj, sj + (cj & 0x7fffffffu), U[ib]);
-                } else {
-                    const uint32_t w = a.ocols[sj + item];
-                    ib = (uint32_t)lower_bound(U, ia + 1, m, w);
-                    hit = ib < m && U[ib] == w;
-                }
-                if (hit) {
-                    atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
-                    atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
-                }
-            }
-        }
+        const uint64_t n = e1 - e0, rem = m - 1 - ia;
+        const bool rev = rem * (1 + bitlen(n)) < n * (1 + bitlen(rem));
+        st = e0;
+        code = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
+        len = rev ? (uint32_t)rem : (uint32_t)n;
     }
-    wave_sync();
-
-    // ---- per-lane depth-first walk from each position-1 candidate ---------
-    uint32_t M[N];
-    uint32_t cur[N], hiv[N];
-    for (uint32_t r1 = lane; r1 < m; r1 += kWave) {
-        M[1] = r1;
-        uint32_t plo, phi;
-        if (N == 3) {
-            cand_range(P, 2, M, m, &plo, &phi);
-            if (plo < phi)
-                for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, 2, M, rows, W, m, w));
-            nodes++;
-            continue;
+    uint32_t incl = len;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= (uint32_t)o) incl += y;
+    }
+    const uint32_t pref = incl - len;
+    const uint32_t T = read_lane(incl, kWave - 1);
+    const uint32_t st_lo = (uint32_t)st, st_hi = (uint32_t)(st >> 32);
+    for (uint32_t f0 = 0; f0 < T; f0 += kWave) {
+        const uint32_t f = f0 + lane;
+        // owner: the last member whose items start at or before f (every lane
+        // runs the same steps, so the lane reads see all lanes active)
+        uint32_t j = 0;
+#pragma unroll
+        for (uint32_t step = 32; step >= 1; step >>= 1) {
+            const uint32_t c = j + step;
+            const uint32_t pc = __shfl(pref, (int)c, kWave);
+            if (c < na && pc <= f) j = c;
         }
-        nodes++;
-        uint32_t level = 2;
-        cand_range(P, 2, M, m, &cur[2], &hiv[2]);
-        while (true) {
-            // next member of cand(level) at or after cur[level]
-            uint32_t pos = cur[level];
-            const uint32_t hi = hiv[level];
-            bool found = false;
-            while (pos < hi) {
-                const uint32_t w = pos >> 6;
-                const uint64_t x = cand_word(P, level, M, rows, W, m, w) & (~0ull << (pos & 63));
-                if (x) {
-                    pos = w * 64 + __builtin_ctzll(x);
-                    found = pos < hi;
-                    break;
-                }
-                pos = (w + 1) * 64;
-            }
-            if (!found) {
-                if (level == 2) break;
-                level--;
-                continue;
-            }
-            cur[level] = pos + 1;
-            M[level] = pos;
-            const uint32_t nx = level + 1;
-            nodes++;
-            cand_range(P, nx, M, m, &plo, &phi);
-            if (nx == N - 1) {
-                if (plo < phi)
-                    for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++)
-                        lane_total += __popcll(cand_word(P, nx, M, rows, W, m, w));
+        const uint32_t item = f - __shfl(pref, (int)j, kWave);
+        const uint32_t cj = __shfl(code, (int)j, kWave);
+        const uint64_t sj = ((uint64_t)__shfl(st_hi, (int)j, kWave) << 32) | __shfl(st_lo, (int)j, kWave);
+        if (f < T) {
+            const uint32_t ia = a0 + j;
+            uint32_t ib;
+            bool hit;
+            if (cj & 0x80000000u) {
+                ib = ia + 1 + item;
+                hit = contains(a.ocols, sj, sj + (cj & 0x7fffffffu), U[ib]);
             } else {
-                cur[nx] = plo;
-                hiv[nx] = phi;
-                level = nx;
+                const uint32_t w = a.ocols[sj + item];
+                ib = (uint32_t)lower_bound(U, ia + 1, m, w);
+                hit = ib < m && U[ib] == w;
+            }
+            if (hit) {
+                atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+                atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
             }
         }
     }
 }
 
+// This lane's depth-first walk of positions level0 .. N-1 given the matched
+// ranks M[1 .. level0-1]: cand(i) is recomputed word by word from the rows
+// (no stacks), the last position is a popcount over its order range.
 template <int N>
-__global__ __launch_bounds__(256) void gm_local_kernel(LocalArgs a) {
+__device__ __forceinline__ void walk_from(const LocalProgram& P, uint32_t* M, uint32_t level0, const uint64_t* rows,
+                                          uint32_t W, uint32_t m, uint64_t& lane_total, uint64_t& nodes) {
+    uint32_t cur[N], hiv[N];
+    uint32_t plo, phi;
+    cand_range(P, level0, M, m, &plo, &phi);
+    if (level0 == N - 1) {
+        if (plo < phi)
+            for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, level0, M, rows, W, m, w));
+        return;
+    }
+    uint32_t level = level0;
+    cur[level] = plo;
+    hiv[level] = phi;
+    while (true) {
+        // next member of cand(level) at or after cur[level]
+        uint32_t pos = cur[level];
+        const uint32_t hi = hiv[level];
+        bool found = false;
+        while (pos < hi) {
+            const uint32_t w = pos >> 6;
+            const uint64_t x = cand_word(P, level, M, rows, W, m, w) & (~0ull << (pos & 63));
+            if (x) {
+                pos = w * 64 + __builtin_ctzll(x);
+                found = pos < hi;
+                break;
+            }
+            pos = (w + 1) * 64;
+        }
+        if (!found) {
+            if (level == level0) break;
+            level--;
+            continue;
+        }
+        cur[level] = pos + 1;
+        M[level] = pos;
+        const uint32_t nx = level + 1;
+        nodes++;
+        cand_range(P, nx, M, m, &plo, &phi);
+        if (nx == N - 1) {
+            if (plo < phi)
+                for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, nx, M, rows, W, m, w));
+        } else {
+            cur[nx] = plo;
+            hiv[nx] = phi;
+            level = nx;
+        }
+    }
+}
+
+// One root on one wave: build the local rows over U, then each lane walks the
+// matches of the position-1 candidates lane, lane+64, ...
+template <int N>
+__device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint32_t m, uint64_t* rows,
+                                           const uint32_t* U, uint32_t* Ucopy, uint64_t& lane_total,
+                                           uint64_t& nodes) {
+    const uint32_t lane = lane_id();
+    const uint32_t W = row_stride(m);
+    for (uint32_t i = lane; i < m * W; i += kWave) rows[i] = 0;
+    if (Ucopy)
+        for (uint32_t i = lane; i < m; i += kWave) Ucopy[i] = a.ocols[lo + i];
+    wave_sync();
+    for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) {
+        build_chunk(a, U, rows, m, W, a0);
+        wave_sync();  // (the chunk's atomics complete before the walk reads rows)
+    }
+    uint32_t M[N];
+    for (uint32_t r1 = lane; r1 < m; r1 += kWave) {
+        M[1] = r1;
+        nodes++;
+        walk_from<N>(a.prog, M, 2, rows, W, m, lane_total, nodes);
+    }
+}
+
+// put root x on the deferred list of the next pass
+__device__ inline void defer_root(const LocalArgs& a, uint32_t x, uint64_t need) {
+    if (a.big_list) {
+        const unsigned long long idx = atomicAdd(&a.ctr[a.defer_ctr], 1ull);
+        a.big_list[idx] = x;
+        atomicMax(&a.ctr[a.defer_ctr + 1], (unsigned long long)need);
+    } else {
+        atomicMax(&a.ctr[LCTR_ERR], (unsigned long long)need);
+    }
+}
+
+// Pass 1 (big_list set): roots whose rows fit the wave's LDS; the others are
+// deferred.  Pass 3 (big_list null): the roots too large for a block's LDS,
+// rows in per-wave global arenas of scratch_words.
+template <int N>
+__global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalArgs a) {
     __shared__ uint64_t lrows[kLocalBlockWaves][kLRowWords];
     __shared__ uint32_t lU[kLocalBlockWaves][kLU];
     const uint32_t lane = lane_id();
@@ -286,20 +327,212 @@ __global__ __launch_bounds__(256) void gm_local_kernel(LocalArgs a) {
             const uint64_t need = local_need(m);
             if (m <= kLU && need <= kLRowWords) {
                 local_root<N>(a, lo, m, lrows[wid], lU[wid], lU[wid], lane_total, nodes);
-                continue;
-            }
-            if (need <= a.scratch_words) {
+            } else if (!a.big_list && need <= a.scratch_words) {
                 local_root<N>(a, lo, m, ws, a.ocols + lo, nullptr, lane_total, nodes);
-                continue;
+            } else if (lane == 0) {
+                defer_root(a, x, need);
             }
-            if (lane == 0) {
-                if (a.big_list) {
-                    const unsigned long long idx = atomicAdd(&a.ctr[LCTR_NBIG], 1ull);
-                    a.big_list[idx] = x;
-                    atomicMax(&a.ctr[LCTR_MAXNEED], (unsigned long long)need);
-                } else {
-                    atomicMax(&a.ctr[LCTR_ERR], (unsigned long long)need);
+        }
+    }
+    const uint64_t tot = wave_sum(lane_total);
+    const uint64_t nd = wave_sum(nodes);
+    if (lane == 0) {
+        if (tot) atomicAdd(&a.ctr[LCTR_COUNT], (unsigned long long)tot);
+        if (nd) atomicAdd(&a.ctr[LCTR_NODES], (unsigned long long)nd);
+    }
+}
+
+// Pass 2: one root per workgroup of 16 waves, its rows in the workgroup's
+// LDS (up to kBRowWords words, |U| <= kBU); roots beyond that are deferred to
+// pass 3.  Waves build 64-member chunks round robin, then take position-1
+// candidates r1 one at a time from an LDS counter; a wave's lanes share the
+// position-2 candidates of its r1, queued 64 at a time in LDS, and each lane
+// walks the rest of its (r1, r2) subtree.
+template <int N>
+__global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
+    __shared__ uint64_t rows[kBRowWords];
+    __shared__ uint32_t U[kBU];
+    __shared__ uint32_t mpref[kBU];   // members' first work item
+    __shared__ uint32_t mstart[kBU];  // members' first entry in ocols
+    __shared__ uint32_t mcode[kBU];   // members' entry count | kRev
+    __shared__ uint32_t queue[kBWaves][kWave];
+    __shared__ uint32_t wsum[kBWaves];
+    __shared__ unsigned long long s_task;
+    __shared__ uint32_t s_next, s_t2;
+    const uint32_t lane = lane_id();
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t nwaves = blockDim.x / kWave;
+    const uint32_t tid = threadIdx.x;
+    const LocalProgram& P = a.prog;
+    uint64_t lane_total = 0, nodes = 0;
+
+    while (true) {
+        __syncthreads();  // every wave is done with the previous root's rows
+        if (threadIdx.x == 0) {
+            s_task = atomicAdd(&a.ctr[LCTR_NEXT], 1ull);
+            s_next = 0;
+        }
+        __syncthreads();
+        const uint64_t t = s_task;
+        if (t >= a.ntasks) break;
+        const uint32_t x = a.list[t];
+        const uint64_t lo = a.orp[x];
+        const uint32_t m = (uint32_t)(a.orp[x + 1] - lo);
+        const uint32_t W = row_stride(m);
+        const uint64_t need = local_need(m);
+        if (m < (uint32_t)(N - 1)) continue;
+        if (m > kBU || need > kBRowWords) {
+            if (threadIdx.x == 0) defer_root(a, x, need);
+            continue;
+        }
+        for (uint32_t i = tid; i < need; i += blockDim.x) rows[i] = 0;
+        for (uint32_t i = tid; i < m; i += blockDim.x) U[i] = a.ocols[lo + i];
+        __syncthreads();
+        // ---- local rows: one work list over all members and threads ---------
+        // member i < m-1 has either n entries of N+(U_i) (restricted to
+        // [U_{i+1}, U_{m-1}]) to look up in U (LDS), or, when its row is very
+        // long, its m-1-i later members to look up in N+(U_i) (global).
+        // Thread t describes members t and t + blockDim.x.
+        uint32_t len[2] = {0, 0};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t i = tid + h * blockDim.x;
+            if (i + 1 < m) {
+                const uint32_t u = U[i];
+                uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
+                if (e1 - e0 > kWave) {
+                    e0 = lower_bound(a.ocols, e0, e1, U[i + 1]);
+                    e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
                 }
+                const uint64_t n = e1 - e0, rem = m - 1 - i;
+                const bool rev = n > (1u << 20) || 8 * rem * (1 + bitlen(n)) < n;
+                mstart[i] = (uint32_t)e0;
+                mcode[i] = rev ? ((uint32_t)n | kRev) : (uint32_t)n;
+                len[h] = rev ? (uint32_t)rem : (uint32_t)n;
+            }
+        }
+        // exclusive prefix over member order: members < blockDim.x first
+        uint32_t incl = len[0];
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, kWave);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        if (lane == kWave - 1) wsum[wid] = incl;
+        __syncthreads();
+        uint32_t before = 0, T = 0;
+        for (uint32_t q = 0; q < nwaves; q++) {
+            const uint32_t ws = wsum[q];
+            before += q < wid ? ws : 0;
+            T += ws;
+        }
+        if (tid < m) mpref[tid] = before + incl - len[0];
+        if (m > blockDim.x && wid == 0) {  // members blockDim.x .. m-1 (fewer than 64)
+            uint32_t incl2 = len[1];
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = __shfl_up(incl2, o, kWave);
+                if (lane >= (uint32_t)o) incl2 += y;
+            }
+            if (tid + blockDim.x < m) mpref[tid + blockDim.x] = T + incl2 - len[1];
+            if (lane == kWave - 1) s_t2 = incl2;
+        }
+        __syncthreads();
+        if (m > blockDim.x) T += s_t2;
+        const uint32_t nm = m - 1;  // members that own items
+        uint32_t hb_own = 1, hb_u = 1;  // highest powers of two <= nm-1 and <= m
+        while (hb_own * 2 <= nm - 1) hb_own *= 2;
+        while (hb_u * 2 <= m) hb_u *= 2;
+        for (uint32_t f0 = 0; f0 < T; f0 += kItems * blockDim.x) {
+            // kItems independent items per thread; every search runs a fixed
+            // number of steps so the items' LDS and global loads overlap
+            uint32_t f[kItems], own[kItems], w[kItems], pos[kItems];
+#pragma unroll
+            for (int q = 0; q < kItems; q++) {
+                f[q] = f0 + q * blockDim.x + tid;
+                own[q] = 0;
+            }
+            for (uint32_t step = hb_own; step >= 1; step >>= 1) {
+#pragma unroll
+                for (int q = 0; q < kItems; q++) {
+                    const uint32_t c = own[q] + step;
+                    if (c < nm && mpref[c] <= f[q]) own[q] = c;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kItems; q++) {
+                const uint32_t code = mcode[own[q]];
+                const uint32_t item = f[q] - mpref[own[q]];
+                w[q] = (f[q] < T && !(code & kRev)) ? a.ocols[(uint64_t)mstart[own[q]] + item] : 0;
+                pos[q] = 0;
+            }
+            for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // pos = |{U_j < w}|
+#pragma unroll
+                for (int q = 0; q < kItems; q++) {
+                    const uint32_t c = pos[q] + step;
+                    if (c <= m && U[c - 1] < w[q]) pos[q] = c;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kItems; q++) {
+                if (f[q] >= T) continue;
+                const uint32_t ia = own[q], code = mcode[ia];
+                uint32_t ib;
+                bool hit;
+                if (code & kRev) {
+                    ib = ia + 1 + (f[q] - mpref[ia]);
+                    hit = contains(a.ocols, mstart[ia], (uint64_t)mstart[ia] + (code & ~kRev), U[ib]);
+                } else {
+                    ib = pos[q];
+                    hit = ib < m && U[ib] == w[q];
+                }
+                if (hit) {
+                    atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+                    atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                }
+            }
+        }
+        __syncthreads();
+
+        uint32_t M[N];
+        while (true) {
+            __builtin_amdgcn_wave_barrier();
+            uint32_t r1 = 0;
+            if (lane == 0) r1 = atomicAdd(&s_next, 1u);
+            r1 = uniform(r1);
+            if (r1 >= m) break;
+            M[1] = r1;
+            if (lane == 0) nodes++;
+            uint32_t plo, phi;
+            cand_range(P, 2, M, m, &plo, &phi);
+            if (plo >= phi) continue;
+            if constexpr (N == 3) {
+                // (cand_word applies the order conditions; the range only bounds the words)
+                for (uint32_t w = (plo >> 6) + lane; w <= (phi - 1) >> 6; w += kWave)
+                    lane_total += __popcll(cand_word(P, 2, M, rows, W, m, w));
+            } else {
+                // position-2 candidates of r1, 64 at a time
+                uint32_t k = 0;
+                auto drain_queue = [&](uint32_t nq) {
+                    wave_sync();
+                    if (lane < nq) {
+                        M[2] = queue[wid][lane];
+                        nodes++;
+                        walk_from<N>(P, M, 3, rows, W, m, lane_total, nodes);
+                    }
+                    wave_sync();
+                };
+                for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) {
+                    const uint64_t cw = cand_word(P, 2, M, rows, W, m, w);
+                    const uint32_t cnt = (uint32_t)__popcll(cw);
+                    if (k + cnt > kWave) {
+                        drain_queue(k);
+                        k = 0;
+                    }
+                    if ((cw >> lane) & 1) queue[wid][k + lanes_below(cw)] = w * 64 + lane;
+                    k += cnt;
+                }
+                if (k) drain_queue(k);
             }
         }
     }
@@ -509,23 +742,30 @@ void gm_local_compile(g2_ctx* c, const g2_plan* p) {
 }
 
 template <int N>
-static void launch_local(uint64_t blocks, uint32_t bw, hipStream_t s, const LocalArgs& a) {
+static void launch_local(bool block, uint64_t blocks, uint32_t bw, hipStream_t s, const LocalArgs& a) {
+    if constexpr (N > 2) {
+        if (block) {
+            hipLaunchKernelGGL(gm_local_block_kernel<N>, dim3(blocks), dim3(kBWaves * kWave), 0, s, a);
+            return;
+        }
+    }
     hipLaunchKernelGGL(gm_local_kernel<N>, dim3(blocks), dim3(bw * kWave), 0, s, a);
 }
 
+// Three passes over the roots: rows in a wave's LDS; rows in a workgroup's
+// LDS (the roots pass 1 deferred); rows in per-wave global arenas sized to
+// the largest need (the roots pass 2 deferred).
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if (c->nv + 1 > 0x7fffffffull) return set_err(c, G2_EINVAL, "local-bitmap matching supports fewer than 2^31-1 vertices");
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kLocalBlockWaves);
     const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM_LOCAL) / bw);
-    const uint64_t nwaves = nblocks * bw;
-    const uint64_t sw = c->tune.wave_scratch_words;
     int rc;
     if ((rc = ensure(c, c->counters, LCTR_N * 8, "counters"))) return rc;
-    if ((rc = ensure(c, c->scratch, nwaves * sw * 8, "wave arenas"))) return rc;
     if ((rc = ensure(c, c->big_list2, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
+    if ((rc = ensure(c, c->list, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
-    float k1 = 0, k2 = 0, tot = 0;
+    float kt[3] = {0, 0, 0}, tot = 0;
     if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
     const uint64_t* orp = nullptr;
     const uint32_t* ocols = nullptr;
@@ -534,66 +774,78 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     LocalArgs a{};
     a.orp = orp;
     a.ocols = ocols;
-    a.list = nullptr;
-    a.ntasks = c->nv;
     a.prog = c->lprog;
-    a.scratch = (uint64_t*)c->scratch.ptr;
-    a.scratch_words = sw;
-    a.big_list = (uint32_t*)c->big_list2.ptr;
     a.ctr = ctr;
-    auto launch = [&](uint64_t blocks) {
+    auto launch = [&](bool block, uint64_t blocks) {
         switch (c->lprog.n) {
-            case 2: launch_local<2>(blocks, bw, c->stream, a); break;
-            case 3: launch_local<3>(blocks, bw, c->stream, a); break;
-            case 4: launch_local<4>(blocks, bw, c->stream, a); break;
-            case 5: launch_local<5>(blocks, bw, c->stream, a); break;
-            case 6: launch_local<6>(blocks, bw, c->stream, a); break;
-            case 7: launch_local<7>(blocks, bw, c->stream, a); break;
-            default: launch_local<8>(blocks, bw, c->stream, a); break;
+            case 2: launch_local<2>(block, blocks, bw, c->stream, a); break;
+            case 3: launch_local<3>(block, blocks, bw, c->stream, a); break;
+            case 4: launch_local<4>(block, blocks, bw, c->stream, a); break;
+            case 5: launch_local<5>(block, blocks, bw, c->stream, a); break;
+            case 6: launch_local<6>(block, blocks, bw, c->stream, a); break;
+            case 7: launch_local<7>(block, blocks, bw, c->stream, a); break;
+            default: launch_local<8>(block, blocks, bw, c->stream, a); break;
         }
         return hipGetLastError();
     };
-    if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
-    if ((e = launch(nblocks))) return hip_err(c, e, "local launch");
-    if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
     unsigned long long h[LCTR_N];
-    if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
-    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
-    (void)hipEventElapsedTime(&k1, c->ev[1], c->ev[2]);
+    auto timed = [&](int pass, bool block, uint64_t blocks) -> int {
+        if ((e = hipMemsetAsync(ctr + LCTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
+        if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
+        if ((e = launch(block, blocks))) return hip_err(c, e, "local launch");
+        if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
+        if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
+        if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
+        (void)hipEventElapsedTime(&kt[pass], c->ev[1], c->ev[2]);
+        return G2_OK;
+    };
+    // pass 1: every root, rows in the wave's LDS
+    a.list = nullptr;
+    a.ntasks = c->nv;
+    a.scratch = nullptr;
+    a.scratch_words = 0;
+    a.big_list = (uint32_t*)c->big_list2.ptr;
+    a.defer_ctr = LCTR_NB1;
+    if ((rc = timed(0, false, nblocks))) return rc;
     uint32_t launches = 1;
-    const uint64_t nbig = h[LCTR_NBIG];
-    if (nbig) {
-        const uint64_t need = h[LCTR_MAXNEED];
-        uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), nbig);
+    const uint64_t n1 = h[LCTR_NB1];
+    uint64_t n2 = 0;
+    if (n1) {  // pass 2: one workgroup per CU
+        a.list = (const uint32_t*)c->big_list2.ptr;
+        a.ntasks = n1;
+        a.big_list = (uint32_t*)c->list.ptr;
+        a.defer_ctr = LCTR_NB2;
+        if ((rc = timed(1, true, std::min<uint64_t>(c->num_cus, n1)))) return rc;
+        launches++;
+        n2 = h[LCTR_NB2];
+    }
+    if (n2) {  // pass 3: per-wave global arenas
+        const uint64_t need = h[LCTR_NEED2];
+        uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), n2);
         while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
         bwaves = (bwaves + bw - 1) / bw * bw;
         if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
-        a.list = (const uint32_t*)c->big_list2.ptr;
-        a.ntasks = nbig;
+        a.list = (const uint32_t*)c->list.ptr;
+        a.ntasks = n2;
         a.scratch = (uint64_t*)c->big_scratch.ptr;
         a.scratch_words = need;
         a.big_list = nullptr;
-        if ((e = hipMemsetAsync(ctr + LCTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
-        if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
-        if ((e = launch(bwaves / bw))) return hip_err(c, e, "local large-root launch");
-        if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
-        launches = 2;
+        if ((rc = timed(2, false, bwaves / bw))) return rc;
+        launches++;
     }
     if ((e = hipEventRecord(c->ev[3], c->stream))) return hip_err(c, e, "event");
-    if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
-    if (nbig) (void)hipEventElapsedTime(&k2, c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[3]);
     if (h[LCTR_ERR]) return set_err(c, G2_EBUDGET, "a root did not fit the large-root arena");
     if (count) *count = h[LCTR_COUNT];
     if (st) {
-        st->kernel_ms = (double)k1 + (double)k2;
+        st->kernel_ms = (double)kt[0] + (double)kt[1] + (double)kt[2];
         st->launches = launches;
         st->total_ms = tot;
         st->nodes = h[LCTR_NODES];
         st->alg_bytes = 0;
-        st->big_roots = nbig;
-        st->max_scratch_words = h[LCTR_MAXNEED];
+        st->big_roots = n1;
+        st->max_scratch_words = n2 ? h[LCTR_NEED2] : h[LCTR_NEED1];
     }
     return G2_OK;
 }

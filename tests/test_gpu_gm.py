@@ -111,19 +111,19 @@ def test_local_path_cliques(g2, orc, gpu_exec):
 
 
 def test_local_path_large_root_pass(g2, orc):
-    """Roots whose local rows exceed the LDS (|U| > 149 here) take the global
-    per-wave arena; with tiny arenas they are deferred to the second launch."""
-    print("case: local path large roots", flush=True)
+    """Roots whose local rows exceed a wave's LDS (|U| > 149 here) are
+    deferred to the workgroup pass (rows in the workgroup's LDS); 3-, 4- and
+    5-cliques, the last walking a per-lane level below the shared r2."""
+    print("case: local path workgroup pass", flush=True)
     g = G.gnp(g2, 400, 0.5, 7)
-    for tuning, deferred in (({}, False), ({"wave_scratch_words": 200, "big_waves": 32}, True)):
-        with g2.Executor(device=0, **tuning) as ex:
-            ex.transfer(g)
-            for q in (0, 24):
-                p = g2.plan_for_preset(q)
-                ex.set_plan(p)
-                got = ex.run_gm()
-                assert (got["stats"]["big_roots"] > 0) == deferred
-                assert got["count"] == orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
+    with g2.Executor(device=0) as ex:
+        ex.transfer(g)
+        for q in (0, 24, 25):
+            p = g2.plan_for_preset(q)
+            ex.set_plan(p)
+            got = ex.run_gm()
+            assert got["stats"]["big_roots"] > 0 and got["stats"]["launches"] == 2
+            assert got["count"] == orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
 
 
 def test_local_path_complete_graph(g2, gpu_exec):
@@ -131,6 +131,20 @@ def test_local_path_complete_graph(g2, gpu_exec):
     k-clique totals are the binomials C(170, k)."""
     print("case: local path complete graph", flush=True)
     gpu_exec.transfer(G.complete(g2, 170))
-    for q, k in ((0, 3), (24, 4)):
+    for q, k in ((0, 3), (24, 4), (25, 5)):
         gpu_exec.set_plan(g2.plan_for_preset(q))
         assert gpu_exec.run_gm()["count"] == math.comb(170, k)
+
+
+def test_local_path_global_arena_pass(g2, gpu_exec):
+    """Turan graph T(1200, 40): degree 1170 everywhere, so in (degree, id)
+    order the first roots have |U| up to 1170, beyond a workgroup's LDS (the
+    third pass, rows in global arenas).  k-cliques pick k of the 40 parts and
+    one of 30 vertices in each: C(40, k) * 30^k."""
+    print("case: local path global-arena pass", flush=True)
+    gpu_exec.transfer(G.turan(g2, [30] * 40))
+    for q, k in ((0, 3), (24, 4)):
+        gpu_exec.set_plan(g2.plan_for_preset(q))
+        got = gpu_exec.run_gm()
+        assert got["stats"]["launches"] == 3
+        assert got["count"] == math.comb(40, k) * 30 ** k
