@@ -59,6 +59,8 @@ constexpr uint32_t kBRowWords = 17664;  // per-workgroup LDS rows (138 KiB): |U|
 constexpr uint32_t kBU = 1040;          // per-workgroup |U| limit (at most blockDim.x + 64)
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
+constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
+constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
 
 struct LocalArgs {
     const uint64_t* __restrict__ orp;   // oriented rows over ranks
@@ -147,9 +149,12 @@ __device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint3
 
 // Local rows of members a0 .. a0+63: pairs (i, j), i < j, with U_j in
 // N+(U_i), both bits set.  One flattened work list keeps all lanes busy:
-// an item is one entry of N+(U_i) (looked up in U by binary search) or one
-// later member U_j (looked up in N+(U_i)), whichever is cheaper for i.
-// rows and U are LDS or global memory (fixed per call site after inlining).
+// an item is one entry of N+(U_i) (looked up in U), or, when that row is
+// longer than (later members) x (its search depth), one later member U_j
+// (looked up in N+(U_i), global memory).
+// Each lane keeps kItemsW items in flight; U is searched by a fixed number
+// of halving steps so the items' loads overlap.  rows and U are LDS or
+// global memory (fixed per call site after inlining).
 __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* U, uint64_t* rows, uint32_t m,
                                             uint32_t W, uint32_t a0) {
     const uint32_t lane = lane_id();
@@ -164,9 +169,9 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
         }
         const uint64_t n = e1 - e0, rem = m - 1 - ia;
-        const bool rev = rem * (1 + bitlen(n)) < n * (1 + bitlen(rem));
+        const bool rev = n > (1u << 20) || rem * (1 + bitlen(n)) < n;
         st = e0;
-        code = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
+        code = rev ? ((uint32_t)n | kRev) : (uint32_t)n;
         len = rev ? (uint32_t)rem : (uint32_t)n;
     }
     uint32_t incl = len;
@@ -178,31 +183,54 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
     const uint32_t pref = incl - len;
     const uint32_t T = read_lane(incl, kWave - 1);
     const uint32_t st_lo = (uint32_t)st, st_hi = (uint32_t)(st >> 32);
-    for (uint32_t f0 = 0; f0 < T; f0 += kWave) {
-        const uint32_t f = f0 + lane;
-        // owner: the last member whose items start at or before f (every lane
+    uint32_t hb_u = 1;  // highest power of two <= m
+    while (hb_u * 2 <= m) hb_u *= 2;
+    for (uint32_t f0 = 0; f0 < T; f0 += kItemsW * kWave) {
+        uint32_t f[kItemsW], j[kItemsW], item[kItemsW], cj[kItemsW], w[kItemsW], pos[kItemsW];
+        uint64_t sj[kItemsW];
+#pragma unroll
+        for (int q = 0; q < kItemsW; q++) {
+            f[q] = f0 + q * kWave + lane;
+            j[q] = 0;
+        }
+        // owners: the last member whose items start at or before f (every lane
         // runs the same steps, so the lane reads see all lanes active)
-        uint32_t j = 0;
 #pragma unroll
         for (uint32_t step = 32; step >= 1; step >>= 1) {
-            const uint32_t c = j + step;
-            const uint32_t pc = __shfl(pref, (int)c, kWave);
-            if (c < na && pc <= f) j = c;
+#pragma unroll
+            for (int q = 0; q < kItemsW; q++) {
+                const uint32_t c = j[q] + step;
+                const uint32_t pc = __shfl(pref, (int)c, kWave);
+                if (c < na && pc <= f[q]) j[q] = c;
+            }
         }
-        const uint32_t item = f - __shfl(pref, (int)j, kWave);
-        const uint32_t cj = __shfl(code, (int)j, kWave);
-        const uint64_t sj = ((uint64_t)__shfl(st_hi, (int)j, kWave) << 32) | __shfl(st_lo, (int)j, kWave);
-        if (f < T) {
-            const uint32_t ia = a0 + j;
+#pragma unroll
+        for (int q = 0; q < kItemsW; q++) {
+            item[q] = f[q] - __shfl(pref, (int)j[q], kWave);
+            cj[q] = __shfl(code, (int)j[q], kWave);
+            sj[q] = ((uint64_t)__shfl(st_hi, (int)j[q], kWave) << 32) | __shfl(st_lo, (int)j[q], kWave);
+            w[q] = (f[q] < T && !(cj[q] & kRev)) ? a.ocols[sj[q] + item[q]] : 0;
+            pos[q] = 0;
+        }
+        for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // pos = |{U_j < w}|
+#pragma unroll
+            for (int q = 0; q < kItemsW; q++) {
+                const uint32_t c = pos[q] + step;
+                if (c <= m && U[c - 1] < w[q]) pos[q] = c;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kItemsW; q++) {
+            if (f[q] >= T) continue;
+            const uint32_t ia = a0 + j[q];
             uint32_t ib;
             bool hit;
-            if (cj & 0x80000000u) {
-                ib = ia + 1 + item;
-                hit = contains(a.ocols, sj, sj + (cj & 0x7fffffffu), U[ib]);
+            if (cj[q] & kRev) {
+                ib = ia + 1 + item[q];
+                hit = contains(a.ocols, sj[q], sj[q] + (cj[q] & ~kRev), U[ib]);
             } else {
-                const uint32_t w = a.ocols[sj + item];
-                ib = (uint32_t)lower_bound(U, ia + 1, m, w);
-                hit = ib < m && U[ib] == w;
+                ib = pos[q];
+                hit = ib < m && U[ib] == w[q];
             }
             if (hit) {
                 atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
@@ -390,8 +418,9 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         __syncthreads();
         // ---- local rows: one work list over all members and threads ---------
         // member i < m-1 has either n entries of N+(U_i) (restricted to
-        // [U_{i+1}, U_{m-1}]) to look up in U (LDS), or, when its row is very
-        // long, its m-1-i later members to look up in N+(U_i) (global).
+        // [U_{i+1}, U_{m-1}]) to look up in U (LDS), or, when n exceeds
+        // (m-1-i) x (1 + log2 n), its m-1-i later members to look up in
+        // N+(U_i) (global).
         // Thread t describes members t and t + blockDim.x.
         uint32_t len[2] = {0, 0};
 #pragma unroll
@@ -405,7 +434,7 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
                     e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
                 }
                 const uint64_t n = e1 - e0, rem = m - 1 - i;
-                const bool rev = n > (1u << 20) || 8 * rem * (1 + bitlen(n)) < n;
+                const bool rev = n > (1u << 20) || rem * (1 + bitlen(n)) < n;
                 mstart[i] = (uint32_t)e0;
                 mcode[i] = rev ? ((uint32_t)n | kRev) : (uint32_t)n;
                 len[h] = rev ? (uint32_t)rem : (uint32_t)n;
@@ -443,20 +472,31 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         uint32_t hb_own = 1, hb_u = 1;  // highest powers of two <= nm-1 and <= m
         while (hb_own * 2 <= nm - 1) hb_own *= 2;
         while (hb_u * 2 <= m) hb_u *= 2;
-        for (uint32_t f0 = 0; f0 < T; f0 += kItems * blockDim.x) {
-            // kItems independent items per thread; every search runs a fixed
-            // number of steps so the items' LDS and global loads overlap
+        constexpr uint32_t kSlice = kItems * kWave;  // consecutive items per wave and round
+        for (uint32_t f0 = wid * kSlice; f0 < T; f0 += nwaves * kSlice) {
+            // the slice's first and last owners (wave-uniform searches, broadcast
+            // reads), then each lane searches only between them
+            const uint32_t flast = min(f0 + kSlice, T) - 1;
+            uint32_t o0 = 0, o1 = 0;
+            for (uint32_t step = hb_own; step >= 1; step >>= 1) {
+                if (o0 + step < nm && mpref[o0 + step] <= f0) o0 += step;
+                if (o1 + step < nm && mpref[o1 + step] <= flast) o1 += step;
+            }
+            uint32_t hb_r = 1;
+            while (hb_r * 2 <= o1 - o0) hb_r *= 2;
             uint32_t f[kItems], own[kItems], w[kItems], pos[kItems];
 #pragma unroll
             for (int q = 0; q < kItems; q++) {
-                f[q] = f0 + q * blockDim.x + tid;
-                own[q] = 0;
+                f[q] = f0 + q * kWave + lane;
+                own[q] = o0;
             }
-            for (uint32_t step = hb_own; step >= 1; step >>= 1) {
+            if (o1 > o0) {
+                for (uint32_t step = hb_r; step >= 1; step >>= 1) {
 #pragma unroll
-                for (int q = 0; q < kItems; q++) {
-                    const uint32_t c = own[q] + step;
-                    if (c < nm && mpref[c] <= f[q]) own[q] = c;
+                    for (int q = 0; q < kItems; q++) {
+                        const uint32_t c = own[q] + step;
+                        if (c <= o1 && mpref[c] <= f[q]) own[q] = c;
+                    }
                 }
             }
 #pragma unroll
@@ -568,7 +608,16 @@ __global__ __launch_bounds__(256) void ocount_kernel(const uint64_t* __restrict_
         const uint32_t x = pi[v];
         const uint64_t e1 = rp[v + 1];
         uint64_t k = 0;
-        for (uint64_t e = rp[v] + lane; e < e1; e += kWave) k += pi[cols[e]] > x;
+        for (uint64_t e0 = rp[v]; e0 < e1; e0 += kRowUnroll * kWave) {  // kRowUnroll loads in flight per lane
+            uint32_t c[kRowUnroll];
+#pragma unroll
+            for (int j = 0; j < kRowUnroll; j++) {
+                const uint64_t e = e0 + j * kWave + lane;
+                c[j] = e < e1 ? cols[e] : 0xffffffffu;
+            }
+#pragma unroll
+            for (int j = 0; j < kRowUnroll; j++) k += c[j] != 0xffffffffu && pi[c[j]] > x;
+        }
         k = wave_sum(k);
         if (lane == 0) cnt[x] = k;
     }
@@ -589,17 +638,22 @@ __global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__
         if (m == 0) continue;
         const uint64_t e1 = rp[v + 1];
         uint64_t k = 0;
-        for (uint64_t e0 = rp[v]; e0 < e1; e0 += kWave) {
-            const uint64_t e = e0 + lane;
-            uint32_t y = 0;
-            bool up = false;
-            if (e < e1) {
-                y = pi[cols[e]];
-                up = y > x;
+        for (uint64_t e0 = rp[v]; e0 < e1; e0 += kRowUnroll * kWave) {
+            uint32_t y[kRowUnroll];
+#pragma unroll
+            for (int j = 0; j < kRowUnroll; j++) {
+                const uint64_t e = e0 + j * kWave + lane;
+                y[j] = e < e1 ? cols[e] : 0xffffffffu;
             }
-            const uint64_t bm = ballot(up);
-            if (up) tmp[base + k + lanes_below(bm)] = y;
-            k += __popcll(bm);
+#pragma unroll
+            for (int j = 0; j < kRowUnroll; j++) y[j] = y[j] != 0xffffffffu ? pi[y[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < kRowUnroll; j++) {
+                const bool up = y[j] > x;
+                const uint64_t bm = ballot(up);
+                if (up) tmp[base + k + lanes_below(bm)] = y[j];
+                k += __popcll(bm);
+            }
         }
         wave_sync();
         for (uint64_t t0 = 0; t0 < m; t0 += kWave) {
@@ -651,7 +705,7 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
     const auto* rp = (const uint64_t*)c->rp.ptr;
     const auto* cols = (const uint32_t*)c->cols.ptr;
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((nv + 255) / 256, 8192));
-    const uint64_t wgrid = (uint64_t)c->num_cus * 16 / 4;
+    const uint64_t wgrid = (uint64_t)c->num_cus * 32 / 4;  // 32 waves per CU: latency-bound row walks
     hipLaunchKernelGGL(okey_kernel, dim3(grid), dim3(256), 0, c->stream, (const uint32_t*)c->deg.ptr, nv, keys);
     size_t tb = t_sort;
     if ((e = hipcub::DeviceRadixSort::SortKeys(base + o_temp, tb, keys, sorted, (int)nv, 0, nbits, c->stream)))
