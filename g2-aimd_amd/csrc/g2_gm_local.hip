@@ -59,6 +59,7 @@ constexpr uint32_t kBRowWords = 17664;  // per-workgroup LDS rows (138 KiB): |U|
 constexpr uint32_t kBU = 1040;          // per-workgroup |U| limit (at most blockDim.x + 64)
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
+constexpr uint32_t kBuckets = 512;      // value buckets of U in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
 constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
 
@@ -386,7 +387,8 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
     __shared__ uint32_t queue[kBWaves][kWave];
     __shared__ uint32_t wsum[kBWaves];
     __shared__ unsigned long long s_task;
-    __shared__ uint32_t s_next, s_t2;
+    __shared__ uint32_t s_next, s_t2, s_maxb;
+    __shared__ uint16_t ubkt[kBuckets + 1];  // first index of U in each value bucket
     const uint32_t lane = lane_id();
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t nwaves = blockDim.x / kWave;
@@ -415,7 +417,23 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         }
         for (uint32_t i = tid; i < need; i += blockDim.x) rows[i] = 0;
         for (uint32_t i = tid; i < m; i += blockDim.x) U[i] = a.ocols[lo + i];
+        if (tid == 0) s_maxb = 0;
         __syncthreads();
+        // value buckets over [U_0, U_{m-1}]: a lookup searches one bucket
+        const uint32_t ulo = U[0];
+        const uint64_t bmul = ((uint64_t)kBuckets << 32) / ((uint64_t)U[m - 1] - ulo + 1);
+        auto bucket = [&](uint32_t v) { return (uint32_t)(((uint64_t)(v - ulo) * bmul) >> 32); };
+        for (uint32_t i = tid; i <= m; i += blockDim.x) {  // buckets (bucket(U_{i-1}), bucket(U_i)] start at i
+            const uint32_t bi = i < m ? bucket(U[i]) : kBuckets;
+            for (uint32_t b = i == 0 ? 0 : bucket(U[i - 1]) + 1; b <= bi; b++) ubkt[b] = (uint16_t)i;
+        }
+        __syncthreads();
+        {
+            uint32_t big = 0;
+            for (uint32_t b = tid; b < kBuckets; b += blockDim.x) big = max<uint32_t>(big, ubkt[b + 1] - ubkt[b]);
+            big = (uint32_t)wave_max((uint64_t)big);
+            if (lane == 0 && big) atomicMax(&s_maxb, big);
+        }
         // ---- local rows: one work list over all members and threads ---------
         // member i < m-1 has either n entries of N+(U_i) (restricted to
         // [U_{i+1}, U_{m-1}]) to look up in U (LDS), or, when n exceeds
@@ -469,9 +487,9 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         __syncthreads();
         if (m > blockDim.x) T += s_t2;
         const uint32_t nm = m - 1;  // members that own items
-        uint32_t hb_own = 1, hb_u = 1;  // highest powers of two <= nm-1 and <= m
+        uint32_t hb_own = 1, hb_u = 1;  // highest powers of two <= nm-1 and <= the largest bucket
         while (hb_own * 2 <= nm - 1) hb_own *= 2;
-        while (hb_u * 2 <= m) hb_u *= 2;
+        while (hb_u * 2 <= s_maxb) hb_u *= 2;
         constexpr uint32_t kSlice = kItems * kWave;  // consecutive items per wave and round
         for (uint32_t f0 = wid * kSlice; f0 < T; f0 += nwaves * kSlice) {
             // the slice's first and last owners (wave-uniform searches, broadcast
@@ -503,14 +521,20 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
             for (int q = 0; q < kItems; q++) {
                 const uint32_t code = mcode[own[q]];
                 const uint32_t item = f[q] - mpref[own[q]];
-                w[q] = (f[q] < T && !(code & kRev)) ? a.ocols[(uint64_t)mstart[own[q]] + item] : 0;
-                pos[q] = 0;
+                w[q] = (f[q] < T && !(code & kRev)) ? a.ocols[(uint64_t)mstart[own[q]] + item] : ulo;
             }
-            for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // pos = |{U_j < w}|
+            uint32_t lim[kItems];
+#pragma unroll
+            for (int q = 0; q < kItems; q++) {  // (an entry above U_{m-1} lands in the last bucket and misses)
+                const uint32_t b = min(bucket(w[q]), kBuckets - 1);
+                pos[q] = ubkt[b];
+                lim[q] = ubkt[b + 1];
+            }
+            for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // pos = |{U_j < w}|, within w's bucket
 #pragma unroll
                 for (int q = 0; q < kItems; q++) {
                     const uint32_t c = pos[q] + step;
-                    if (c <= m && U[c - 1] < w[q]) pos[q] = c;
+                    if (c <= lim[q] && U[c - 1] < w[q]) pos[q] = c;
                 }
             }
 #pragma unroll
