@@ -87,6 +87,16 @@ __device__ __forceinline__ uint64_t fetch_task(unsigned long long* ctr) {
     return uniform((uint64_t)t);
 }
 
+// First of k consecutive tasks (same wave barrier as fetch_task).  One
+// device-scope atomic per fetch: a counter takes roughly one atomic per 10 ns,
+// so millions of cheap tasks fetched one at a time queue behind it.
+__device__ __forceinline__ uint64_t fetch_tasks(unsigned long long* ctr, uint32_t k) {
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long t = 0;
+    if (lane_id() == 0) t = atomicAdd(ctr, (unsigned long long)k);
+    return uniform((uint64_t)t);
+}
+
 // first index in [lo,hi) whose value is >= val (hi if none)
 __device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ cols, uint64_t lo, uint64_t hi, uint32_t val) {
     while (lo < hi) {

@@ -62,6 +62,7 @@ constexpr int kItems = 8;               // work items per thread in flight in th
 constexpr uint32_t kBuckets = 512;      // value buckets of U in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
 constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
+constexpr uint32_t kFetchRun = 32;      // roots per counter fetch in the first pass
 
 struct LocalArgs {
     const uint64_t* __restrict__ orp;   // oriented rows over ranks
@@ -165,12 +166,18 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
     if (lane < na) {
         const uint32_t ia = a0 + lane, u = U[ia];
         uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
-        if (e1 - e0 > kWave) {  // restrict a long row to [U[ia+1], U[m-1]]
+        const uint64_t rem = m - 1 - ia;
+        // a long row whose later members are few is searched whole (no
+        // restricting searches on the chain); otherwise it is restricted to
+        // [U[ia+1], U[m-1]] and the cheaper side chosen again
+        bool rev = rem * (1 + bitlen(e1 - e0)) < e1 - e0;
+        if (!rev && e1 - e0 > kWave) {
             e0 = lower_bound(a.ocols, e0, e1, U[ia + 1]);
             e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
+            rev = rem * (1 + bitlen(e1 - e0)) < e1 - e0;
         }
-        const uint64_t n = e1 - e0, rem = m - 1 - ia;
-        const bool rev = n > (1u << 20) || rem * (1 + bitlen(n)) < n;
+        const uint64_t n = e1 - e0;
+        rev = rev || n > (1u << 20);
         st = e0;
         code = rev ? ((uint32_t)n | kRev) : (uint32_t)n;
         len = rev ? (uint32_t)rem : (uint32_t)n;
@@ -205,19 +212,49 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
                 if (c < na && pc <= f[q]) j[q] = c;
             }
         }
+        uint32_t hb_r = 0;  // largest search range among this lane's lookups
 #pragma unroll
         for (int q = 0; q < kItemsW; q++) {
             item[q] = f[q] - __shfl(pref, (int)j[q], kWave);
             cj[q] = __shfl(code, (int)j[q], kWave);
             sj[q] = ((uint64_t)__shfl(st_hi, (int)j[q], kWave) << 32) | __shfl(st_lo, (int)j[q], kWave);
-            w[q] = (f[q] < T && !(cj[q] & kRev)) ? a.ocols[sj[q] + item[q]] : 0;
+            const bool valid = f[q] < T;
+            if (valid && (cj[q] & kRev)) {
+                w[q] = U[a0 + j[q] + 1 + item[q]];  // the later member to look up
+                hb_r = max(hb_r, cj[q] & ~kRev);
+            } else {
+                w[q] = valid ? a.ocols[sj[q] + item[q]] : 0;
+            }
             pos[q] = 0;
         }
-        for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // pos = |{U_j < w}|
+        for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // stream: pos = |{U_j < w}|
 #pragma unroll
             for (int q = 0; q < kItemsW; q++) {
                 const uint32_t c = pos[q] + step;
-                if (c <= m && U[c - 1] < w[q]) pos[q] = c;
+                if (!(cj[q] & kRev) && c <= m && U[c - 1] < w[q]) pos[q] = c;
+            }
+        }
+        // lookups: rpos = |{entries <= w}| in the member's row, with the entry
+        // at rpos-1 kept from the step that accepted it
+        uint32_t rpos[kItemsW], last[kItemsW];
+#pragma unroll
+        for (int q = 0; q < kItemsW; q++) {
+            rpos[q] = 0;
+            last[q] = ~0u;
+        }
+        uint32_t step0 = 1;
+        while (step0 * 2 <= hb_r) step0 *= 2;
+        for (uint32_t step = hb_r ? step0 : 0; step >= 1; step >>= 1) {
+#pragma unroll
+            for (int q = 0; q < kItemsW; q++) {
+                const uint32_t c = rpos[q] + step;
+                if ((cj[q] & kRev) && c <= (cj[q] & ~kRev)) {
+                    const uint32_t v = a.ocols[sj[q] + c - 1];
+                    if (v <= w[q]) {
+                        rpos[q] = c;
+                        last[q] = v;
+                    }
+                }
             }
         }
 #pragma unroll
@@ -228,7 +265,7 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             bool hit;
             if (cj[q] & kRev) {
                 ib = ia + 1 + item[q];
-                hit = contains(a.ocols, sj[q], sj[q] + (cj[q] & ~kRev), U[ib]);
+                hit = rpos[q] > 0 && last[q] == w[q];
             } else {
                 ib = pos[q];
                 hit = ib < m && U[ib] == w[q];
@@ -342,10 +379,19 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
     uint64_t* ws = a.scratch + gwave * a.scratch_words;
     uint64_t lane_total = 0, nodes = 0;
 
+    // roots come in runs of kFetchRun from the counter in the first pass (most
+    // of them are trivial, and one atomic per root would queue the waves on
+    // the counter), singly in the third (few, heavy roots)
+    const uint32_t run = a.big_list ? kFetchRun : 1;
+    uint64_t t = 0, tend = 0;
     while (true) {
-        const uint64_t t = fetch_task(&a.ctr[LCTR_NEXT]);
+        if (t == tend) {
+            t = fetch_tasks(&a.ctr[LCTR_NEXT], run);
+            tend = min<uint64_t>(t + run, a.ntasks);
+        }
         if (t >= a.ntasks) break;
         const uint32_t x = uniform(a.list ? a.list[t] : (uint32_t)t);
+        t++;
         const uint64_t lo = uniform(a.orp[x]);
         const uint32_t m = (uint32_t)(uniform(a.orp[x + 1]) - lo);
         if (m < (uint32_t)(N - 1)) continue;
