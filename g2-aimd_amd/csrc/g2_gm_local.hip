@@ -84,41 +84,25 @@ __host__ __device__ inline uint32_t row_stride(uint64_t m) { return (uint32_t)((
 // u64 words of a root's local rows
 __host__ __device__ inline uint64_t local_need(uint64_t m) { return m * row_stride(m); }
 
-__device__ inline uint64_t bits_above(uint64_t r, uint64_t w) {  // bits > r within word w
-    const uint64_t lo = w * 64;
-    if (r + 1 <= lo) return ~0ull;
-    if (r >= lo + 63) return 0;
-    return ~0ull << (r - lo + 1);
+// bits of word w at positions in [lo, hi)
+__device__ inline uint64_t range_word(uint32_t lo, uint32_t hi, uint32_t w) {
+    const uint32_t base = w * 64;
+    const uint64_t a = lo <= base ? ~0ull : (lo >= base + 64 ? 0 : ~0ull << (lo - base));
+    const uint64_t b = hi >= base + 64 ? ~0ull : (hi <= base ? 0 : (1ull << (hi - base)) - 1);
+    return a & b;
 }
 
-__device__ inline uint64_t bits_below(uint64_t r, uint64_t w) {  // bits < r within word w
-    const uint64_t lo = w * 64;
-    if (r <= lo) return 0;
-    if (r >= lo + 64) return ~0ull;
-    return (1ull << (r - lo)) - 1;
-}
-
-// word w of the candidates of position i given the matched ranks M[1..i-1]
+// word w of the candidates of position i given the matched ranks M[1..i-1]:
+// the back neighbours' rows, within [lo, hi) (the order conditions' range
+// from cand_range, which also keeps the bits below m), minus NON_EQUAL holes
 __device__ inline uint64_t cand_word(const LocalProgram& P, uint32_t i, const uint32_t* M,
-                                     const uint64_t* rows, uint32_t W, uint32_t m, uint32_t w) {
-    uint64_t x = (w + 1) * 64 <= m ? ~0ull : ((1ull << (m - w * 64)) - 1);
+                                     const uint64_t* rows, uint32_t W, uint32_t lo, uint32_t hi, uint32_t w) {
+    uint64_t x = range_word(lo, hi, w);
     uint32_t b = P.bn_mask[i];
     while (b) {
         const uint32_t j = __builtin_ctz(b);
         b &= b - 1;
         x &= rows[(uint64_t)M[j] * W + w];
-    }
-    uint32_t g = P.gt_mask[i];
-    while (g) {
-        const uint32_t j = __builtin_ctz(g);
-        g &= g - 1;
-        x &= bits_above(M[j], w);
-    }
-    uint32_t l = P.lt_mask[i];
-    while (l) {
-        const uint32_t j = __builtin_ctz(l);
-        l &= l - 1;
-        x &= bits_below(M[j], w);
     }
     uint32_t q = P.ne_mask[i];
     while (q) {
@@ -289,7 +273,7 @@ __device__ __forceinline__ void walk_from(const LocalProgram& P, uint32_t* M, ui
     cand_range(P, level0, M, m, &plo, &phi);
     if (level0 == N - 1) {
         if (plo < phi)
-            for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, level0, M, rows, W, m, w));
+            for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, level0, M, rows, W, plo, phi, w));
         return;
     }
     uint32_t level = level0;
@@ -302,7 +286,7 @@ __device__ __forceinline__ void walk_from(const LocalProgram& P, uint32_t* M, ui
         bool found = false;
         while (pos < hi) {
             const uint32_t w = pos >> 6;
-            const uint64_t x = cand_word(P, level, M, rows, W, m, w) & (~0ull << (pos & 63));
+            const uint64_t x = cand_word(P, level, M, rows, W, 0, hi, w) & (~0ull << (pos & 63));  // (pos >= the range's low end)
             if (x) {
                 pos = w * 64 + __builtin_ctzll(x);
                 found = pos < hi;
@@ -322,7 +306,7 @@ __device__ __forceinline__ void walk_from(const LocalProgram& P, uint32_t* M, ui
         cand_range(P, nx, M, m, &plo, &phi);
         if (nx == N - 1) {
             if (plo < phi)
-                for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, nx, M, rows, W, m, w));
+                for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) lane_total += __popcll(cand_word(P, nx, M, rows, W, plo, phi, w));
         } else {
             cur[nx] = plo;
             hiv[nx] = phi;
@@ -617,9 +601,8 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
             cand_range(P, 2, M, m, &plo, &phi);
             if (plo >= phi) continue;
             if constexpr (N == 3) {
-                // (cand_word applies the order conditions; the range only bounds the words)
                 for (uint32_t w = (plo >> 6) + lane; w <= (phi - 1) >> 6; w += kWave)
-                    lane_total += __popcll(cand_word(P, 2, M, rows, W, m, w));
+                    lane_total += __popcll(cand_word(P, 2, M, rows, W, plo, phi, w));
             } else {
                 // position-2 candidates of r1, 64 at a time
                 uint32_t k = 0;
@@ -633,7 +616,7 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
                     wave_sync();
                 };
                 for (uint32_t w = plo >> 6; w <= (phi - 1) >> 6; w++) {
-                    const uint64_t cw = cand_word(P, 2, M, rows, W, m, w);
+                    const uint64_t cw = cand_word(P, 2, M, rows, W, plo, phi, w);
                     const uint32_t cnt = (uint32_t)__popcll(cw);
                     if (k + cnt > kWave) {
                         drain_queue(k);

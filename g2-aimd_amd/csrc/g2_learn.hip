@@ -38,10 +38,18 @@ struct FeatArgs {
 
 __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
     const uint32_t lane = lane_id();
+    // sources come in runs of 16 per counter fetch: a root's features cost a
+    // few row walks, and one atomic per root would queue the waves on it
+    constexpr uint32_t kRun = 16;
+    uint64_t t = 0, tend = 0;
     while (true) {
-        const uint64_t t = fetch_task(a.next);
+        if (t == tend) {
+            t = fetch_tasks(a.next, kRun);
+            tend = min<uint64_t>(t + kRun, a.ns);
+        }
         if (t >= a.ns) break;
-        const uint32_t v = uniform(a.src[t]);
+        const uint64_t ti = t++;
+        const uint32_t v = uniform(a.src[ti]);
         const uint64_t lo = a.rp[v], hi = a.rp[v + 1], d = hi - lo;
         uint64_t two = 0, p0 = 0, tri2 = 0, shared = 0;
         for (uint64_t e = lo + lane; e < hi; e += kWave) {
@@ -65,7 +73,7 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
         two = wave_sum(two);
         p0 = wave_sum(p0);
         if (lane == 0) {
-            int64_t* o = a.out + t * G2_NFEAT_INT;
+            int64_t* o = a.out + ti * G2_NFEAT_INT;
             o[0] = (int64_t)d;
             o[1] = (int64_t)two;
             o[2] = (int64_t)(tri2 / 2);
