@@ -651,47 +651,19 @@ __global__ __launch_bounds__(256) void opi_kernel(const unsigned long long* __re
         pi[(uint32_t)sorted[i]] = (uint32_t)i;
 }
 
-// out-degree of every rank: one wave per vertex
+// Out-degree of every rank, with the above-neighbours' ranks of vertex v
+// compacted into tmp[rp[v] ..] (row order): one wave per vertex, kRowUnroll
+// row entries per lane in flight.
 __global__ __launch_bounds__(256) void ocount_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols,
                                                      const uint32_t* __restrict__ pi, uint64_t nv,
-                                                     unsigned long long* __restrict__ cnt) {
-    const uint32_t lane = lane_id();
-    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
-    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < nv; v += nw) {
-        const uint32_t x = pi[v];
-        const uint64_t e1 = rp[v + 1];
-        uint64_t k = 0;
-        for (uint64_t e0 = rp[v]; e0 < e1; e0 += kRowUnroll * kWave) {  // kRowUnroll loads in flight per lane
-            uint32_t c[kRowUnroll];
-#pragma unroll
-            for (int j = 0; j < kRowUnroll; j++) {
-                const uint64_t e = e0 + j * kWave + lane;
-                c[j] = e < e1 ? cols[e] : 0xffffffffu;
-            }
-#pragma unroll
-            for (int j = 0; j < kRowUnroll; j++) k += c[j] != 0xffffffffu && pi[c[j]] > x;
-        }
-        k = wave_sum(k);
-        if (lane == 0) cnt[x] = k;
-    }
-}
-
-// N+(x), sorted: compact the above-neighbours' ranks into tmp, then place each
-// at its rank within the row (ties, which a simple graph has none of, by slot)
-__global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols,
-                                                    const uint32_t* __restrict__ pi, uint64_t nv,
-                                                    const unsigned long long* __restrict__ orp,
-                                                    uint32_t* __restrict__ tmp, uint32_t* __restrict__ ocols) {
+                                                     unsigned long long* __restrict__ cnt, uint32_t* __restrict__ tmp) {
     const uint32_t lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
     for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < nv; v += nw) {
         const uint32_t x = uniform(pi[v]);
-        const uint64_t base = uniform((uint64_t)orp[x]);
-        const uint64_t m = uniform((uint64_t)orp[x + 1]) - base;
-        if (m == 0) continue;
-        const uint64_t e1 = rp[v + 1];
+        const uint64_t e00 = uniform(rp[v]), e1 = uniform(rp[v + 1]);
         uint64_t k = 0;
-        for (uint64_t e0 = rp[v]; e0 < e1; e0 += kRowUnroll * kWave) {
+        for (uint64_t e0 = e00; e0 < e1; e0 += kRowUnroll * kWave) {
             uint32_t y[kRowUnroll];
 #pragma unroll
             for (int j = 0; j < kRowUnroll; j++) {
@@ -704,17 +676,70 @@ __global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__
             for (int j = 0; j < kRowUnroll; j++) {
                 const bool up = y[j] > x;
                 const uint64_t bm = ballot(up);
-                if (up) tmp[base + k + lanes_below(bm)] = y[j];
+                if (up) tmp[e00 + k + lanes_below(bm)] = y[j];
                 k += __popcll(bm);
             }
         }
-        wave_sync();
+        if (lane == 0) cnt[x] = k;
+    }
+}
+
+// N+(x), sorted.  A row of at most 64 ranks: each goes to its rank within
+// the row, counted over the lanes; up to kSortMax: bitonic sort in the wave's
+// LDS; longer: ranks counted chunk by chunk (ties, which a simple graph has
+// none of, by slot).
+constexpr uint32_t kSortMax = 1024;
+__global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ pi,
+                                                    uint64_t nv, const unsigned long long* __restrict__ orp,
+                                                    const uint32_t* __restrict__ tmp, uint32_t* __restrict__ ocols) {
+    __shared__ uint32_t sbuf[4][kSortMax];
+    uint32_t* buf = sbuf[threadIdx.x / kWave];
+    const uint32_t lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < nv; v += nw) {
+        const uint32_t x = uniform(pi[v]);
+        const uint64_t base = uniform((uint64_t)orp[x]);
+        const uint64_t m = uniform((uint64_t)orp[x + 1]) - base;
+        if (m == 0) continue;
+        const uint32_t* src = tmp + uniform(rp[v]);
+        if (m <= kWave) {  // one value per lane: rank by lane reads
+            const uint32_t my = lane < m ? src[lane] : 0xffffffffu;
+            uint32_t r = 0;
+            for (uint32_t q = 0; q < (uint32_t)m; q++) {
+                const uint32_t o = read_lane(my, q);
+                r += o < my || (o == my && q < lane);
+            }
+            if (lane < m) ocols[base + r] = my;
+            continue;
+        }
+        if (m <= kSortMax) {
+            uint32_t P = 2 * kWave;
+            while (P < m) P <<= 1;
+            for (uint32_t i = lane; i < P; i += kWave) buf[i] = i < m ? src[i] : 0xffffffffu;
+            wave_sync();
+            for (uint32_t k = 2; k <= P; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t t = lane; t < P / 2; t += kWave) {
+                        const uint32_t i = 2 * t - (t & (j - 1)), l = i + j;  // i has bit j clear
+                        const uint32_t a0 = buf[i], a1 = buf[l];
+                        if ((a0 > a1) == ((i & k) == 0)) {
+                            buf[i] = a1;
+                            buf[l] = a0;
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+            for (uint32_t i = lane; i < m; i += kWave) ocols[base + i] = buf[i];
+            wave_sync();  // the buffer is refilled by the next row
+            continue;
+        }
         for (uint64_t t0 = 0; t0 < m; t0 += kWave) {
             const uint64_t i = t0 + lane;
-            const uint32_t my = i < m ? tmp[base + i] : 0xffffffffu;
+            const uint32_t my = i < m ? src[i] : 0xffffffffu;
             uint64_t r = 0;
             for (uint64_t c0 = 0; c0 < m; c0 += kWave) {
-                const uint32_t vc = c0 + lane < m ? tmp[base + c0 + lane] : 0xffffffffu;
+                const uint32_t vc = c0 + lane < m ? src[c0 + lane] : 0xffffffffu;
                 const uint32_t nc = (uint32_t)min<uint64_t>(kWave, m - c0);
                 for (uint32_t q = 0; q < nc; q++) {
                     const uint32_t o = read_lane(vc, q);
@@ -765,11 +790,11 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
         return hip_err(c, e, "rank sort");
     hipLaunchKernelGGL(opi_kernel, dim3(grid), dim3(256), 0, c->stream, sorted, nv, pi);
     if ((e = hipMemsetAsync(cnt + nv, 0, 8, c->stream))) return hip_err(c, e, "memset count");
-    hipLaunchKernelGGL(ocount_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, pi, nv, cnt);
+    hipLaunchKernelGGL(ocount_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, pi, nv, cnt, tmp);
     tb = t_scan;
     if ((e = hipcub::DeviceScan::ExclusiveSum(base + o_temp, tb, cnt, d_orp, (int)(nv + 1), c->stream)))
         return hip_err(c, e, "oriented scan");
-    hipLaunchKernelGGL(ofill_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, pi, nv, d_orp, tmp, d_ocols);
+    hipLaunchKernelGGL(ofill_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, pi, nv, d_orp, tmp, d_ocols);
     if ((e = hipGetLastError())) return hip_err(c, e, "oriented rows kernels");
     *orp = (const uint64_t*)d_orp;
     *ocols = d_ocols;
