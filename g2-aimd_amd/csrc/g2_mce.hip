@@ -219,6 +219,8 @@ __device__ inline uint64_t claim_record(const MceArgs& a, uint64_t total_tasks) 
 
 constexpr int kMaxBlockWaves = 4;  // __launch_bounds__(256)
 constexpr int kScoreWords = 2;     // candidate words scored per round (arena path)
+constexpr int kLdsArena = 896;     // u64 words of LDS per wave for the walk of a medium root
+constexpr int kMinLdsFrames = 16; // frames that must fit next to the rows for the LDS layout
 
 // LDS of one wave for the register-resident path
 struct SmallLds {
@@ -428,12 +430,14 @@ template <bool ACCOUNT>
 __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
     __shared__ unsigned long long lhist[64];
     __shared__ SmallLds small_lds[kMaxBlockWaves];
+    __shared__ uint64_t lds_arena[kMaxBlockWaves][kLdsArena];
     for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x) lhist[i] = 0;
     __syncthreads();
 
     const uint32_t lane = lane_id();
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     SmallLds& SL = small_lds[threadIdx.x / kWave];
+    uint64_t* SA = lds_arena[threadIdx.x / kWave];
     uint64_t* ws = a.scratch + gwave * a.scratch_words;
     uint64_t my_cliques = 0, my_nodes = 0, my_bytes = 0;
     uint64_t small_cliques = 0, small_hist = 0, small_big = 0;  // register-resident path tallies
@@ -494,68 +498,200 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
 
         // ---- build the neighbourhood bitsets ------------------------------
         for (uint64_t i = lane; i < d * W + p0 * WX; i += kWave) ws[i] = 0;
+        // C is kept in the wave's LDS arena for the build when it fits (the
+        // arena is free until the walk's layout is copied in)
+        const bool ldsC = d <= 2 * (uint64_t)kLdsArena;
+        const uint32_t* Cb = C;
+        if (ldsC) {
+            uint32_t* Cl = (uint32_t*)SA;
+            for (uint64_t i = lane; i < d; i += kWave) Cl[i] = C[i];
+            Cb = Cl;
+        }
         arena_sync();
-        // Chunks of 64 later neighbours; within a chunk one flattened work list
-        // of (neighbour, item) pairs keeps all lanes busy with independent
-        // loads: an item is either one entry of the neighbour's row slice
-        // (located in C by binary search) or one position of C (looked up in
-        // the slice), whichever of the two is cheaper for that neighbour.
-        for (uint64_t j0 = 0; j0 < p0; j0 += kWave) {
-            const uint32_t nj = (uint32_t)min<uint64_t>(kWave, p0 - j0);
-            uint32_t len = 0;
-            if (lane < nj) {
-                const uint32_t p = C[s + j0 + lane];
-                uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
-                if (e1 - e0 > kWave) {  // restrict a long row to [C[0], C[d-1]]
-                    e0 = lower_bound(a.cols, e0, e1, C[0]);
-                    e1 = lower_bound(a.cols, e0, e1, C[d - 1] + 1);
+        if (ldsC) {
+            // Oriented build: every edge inside C that touches P0 is found once,
+            // in the later part of its lower end's row.  Member k (vertex c)
+            // streams N(c) within [lo, C[d-1]], lo = v+1 for c in X0 (partners
+            // in P0) and c+1 for c in P0 (later partners), and looks each entry
+            // up among C's P0 part in LDS.  These later-neighbour parts are
+            // short in degeneracy order.
+            uint32_t hb_p = 1;  // highest power of two <= p0
+            while ((uint64_t)hb_p * 2 <= p0) hb_p *= 2;
+            for (uint64_t k0 = 0; k0 < d; k0 += kWave) {
+                const uint32_t nk = (uint32_t)min<uint64_t>(kWave, d - k0);
+                uint32_t len = 0;
+                if (lane < nk) {
+                    const uint64_t k = k0 + lane;
+                    const uint32_t c = Cb[k];
+                    const uint64_t e0 = lower_bound(a.cols, a.rp[c], a.rp[c + 1], k < s ? v + 1 : c + 1);
+                    const uint64_t e1 = lower_bound(a.cols, e0, a.rp[c + 1], Cb[d - 1] + 1);
+                    SL.start[lane] = e0;
+                    len = (uint32_t)(e1 - e0);
                 }
-                const uint64_t n = e1 - e0;
-                const bool rev = d * (1 + bitlen(n)) < n * (1 + bitlen(d));
-                SL.start[lane] = e0;
-                SL.rlen[lane] = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
-                len = rev ? (uint32_t)d : (uint32_t)n;
+                uint32_t incl = len;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t y = __shfl_up(incl, o, kWave);
+                    if (lane >= (uint32_t)o) incl += y;
+                }
+                SL.pref[lane] = incl - len;
+                const uint32_t T = read_lane(incl, kWave - 1);
+                wave_sync();
+                for (uint32_t f = lane; f < T; f += kWave) {
+                    uint32_t lo2 = 0, hi2 = nk;
+                    while (hi2 - lo2 > 1) {
+                        const uint32_t mid = (lo2 + hi2) >> 1;
+                        if (SL.pref[mid] <= f) lo2 = mid; else hi2 = mid;
+                    }
+                    const uint64_t k = k0 + lo2;
+                    const uint32_t w = a.cols[SL.start[lo2] + (f - SL.pref[lo2])];
+                    uint64_t pos = s;  // first P0 position with C >= w, fixed halving steps
+                    for (uint32_t step = hb_p; step >= 1; step >>= 1)
+                        if (pos + step <= d && Cb[pos + step - 1] < w) pos += step;
+                    if (pos < d && Cb[pos] == w) {
+                        const uint64_t j2 = pos - s;
+                        if (k < s) {  // X0 member k, P0 member j2
+                            atomicOr((unsigned long long*)&adjX[j2 * WX + (k >> 6)], 1ull << (k & 63));
+                            atomicOr((unsigned long long*)&adjP[k * W + (j2 >> 6)], 1ull << (j2 & 63));
+                        } else {      // P0 members j1 < j2
+                            const uint64_t j1 = k - s;
+                            atomicOr((unsigned long long*)&adjP[(s + j1) * W + (j2 >> 6)], 1ull << (j2 & 63));
+                            atomicOr((unsigned long long*)&adjP[(s + j2) * W + (j1 >> 6)], 1ull << (j1 & 63));
+                        }
+                    }
+                }
+                wave_sync();  // the chunk's work list is rewritten next
             }
-            uint32_t incl = len;
+        } else {
+            // C in global memory: chunks of 64 later neighbours; within a chunk
+            // one flattened work list of (neighbour, item) pairs keeps all lanes
+            // busy with independent loads: an item is either one entry of the
+            // neighbour's row slice (located in C by binary search) or one
+            // position of C (looked up in the slice), whichever is cheaper.
+            for (uint64_t j0 = 0; j0 < p0; j0 += kWave) {
+                const uint32_t nj = (uint32_t)min<uint64_t>(kWave, p0 - j0);
+                uint32_t len = 0;
+                if (lane < nj) {
+                    const uint32_t p = C[s + j0 + lane];
+                    uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
+                    if (e1 - e0 > kWave) {  // restrict a long row to [C[0], C[d-1]]
+                        e0 = lower_bound(a.cols, e0, e1, C[0]);
+                        e1 = lower_bound(a.cols, e0, e1, C[d - 1] + 1);
+                    }
+                    const uint64_t n = e1 - e0;
+                    const bool rev = d * (1 + bitlen(n)) < n * (1 + bitlen(d));
+                    SL.start[lane] = e0;
+                    SL.rlen[lane] = rev ? ((uint32_t)n | 0x80000000u) : (uint32_t)n;
+                    len = rev ? (uint32_t)d : (uint32_t)n;
+                }
+                uint32_t incl = len;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t y = __shfl_up(incl, o, kWave);
+                    if (lane >= (uint32_t)o) incl += y;
+                }
+                SL.pref[lane] = incl - len;
+                const uint32_t T = read_lane(incl, kWave - 1);
+                wave_sync();
+                for (uint32_t f = lane; f < T; f += kWave) {
+                    uint32_t lo2 = 0, hi2 = nj;
+                    while (hi2 - lo2 > 1) {
+                        const uint32_t mid = (lo2 + hi2) >> 1;
+                        if (SL.pref[mid] <= f) lo2 = mid; else hi2 = mid;
+                    }
+                    const uint64_t j = j0 + lo2;
+                    const uint32_t rl = SL.rlen[lo2], item = f - SL.pref[lo2];
+                    uint64_t k;
+                    bool hit;
+                    if (rl & 0x80000000u) {
+                        k = item;
+                        hit = contains(a.cols, SL.start[lo2], SL.start[lo2] + (rl & 0x7fffffffu), C[k]);
+                    } else {
+                        const uint32_t w = a.cols[SL.start[lo2] + item];
+                        k = w < v ? lower_bound(C, 0, s, w) : lower_bound(C, s, d, w);
+                        hit = k < d && C[k] == w;
+                    }
+                    if (hit) {
+                        if (k < s) {
+                            atomicOr((unsigned long long*)&adjX[j * WX + (k >> 6)], 1ull << (k & 63));
+                            atomicOr((unsigned long long*)&adjP[k * W + (j >> 6)], 1ull << (j & 63));
+                        } else {
+                            const uint64_t kk = k - s;
+                            atomicOr((unsigned long long*)&adjP[(s + j) * W + (kk >> 6)], 1ull << (kk & 63));
+                        }
+                    }
+                }
+                wave_sync();  // the chunk's work list is rewritten next
+            }
+        }
+        arena_sync();
+
+        // ---- the walk's layout --------------------------------------------
+        // X0 members without a neighbour in P0 can neither be the pivot (they
+        // score 0, and a P member with a larger position always exists where
+        // a pivot is chosen) nor survive into a child's X, so the walk keeps
+        // only X' = the others.  When the rows over X' and at least
+        // kMinLdsFrames frames fit the wave's LDS, they are copied there; the
+        // frames deeper than what fits stay in the global arena.  The tree,
+        // pivots and children are unchanged.  (The byte model keeps the
+        // reference's full X.)
+        uint64_t sE = s, WXE = WX, FWE = FW, DL = 0;
+        uint64_t* aP = adjP;
+        uint64_t* aX = adjX;
+        uint64_t* lfr = frames;
+        if (!ACCOUNT && WX <= kWave) {
+            uint64_t xmw = 0;  // lane t: X word t of the members with a P0 neighbour
+            if (lane < WX)
+                for (uint64_t j = 0; j < p0; j++) xmw |= adjX[j * WX + lane];
+            const uint32_t cnt = __popcll(xmw);
+            uint32_t incl = cnt;
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
                 const uint32_t y = __shfl_up(incl, o, kWave);
                 if (lane >= (uint32_t)o) incl += y;
             }
-            SL.pref[lane] = incl - len;
-            const uint32_t T = read_lane(incl, kWave - 1);
-            wave_sync();
-            for (uint32_t f = lane; f < T; f += kWave) {
-                uint32_t lo2 = 0, hi2 = nj;
-                while (hi2 - lo2 > 1) {
-                    const uint32_t mid = (lo2 + hi2) >> 1;
-                    if (SL.pref[mid] <= f) lo2 = mid; else hi2 = mid;
-                }
-                const uint64_t j = j0 + lo2;
-                const uint32_t rl = SL.rlen[lo2], item = f - SL.pref[lo2];
-                uint64_t k;
-                bool hit;
-                if (rl & 0x80000000u) {
-                    k = item;
-                    hit = contains(a.cols, SL.start[lo2], SL.start[lo2] + (rl & 0x7fffffffu), C[k]);
-                } else {
-                    const uint32_t w = a.cols[SL.start[lo2] + item];
-                    k = w < v ? lower_bound(C, 0, s, w) : lower_bound(C, s, d, w);
-                    hit = k < d && C[k] == w;
-                }
-                if (hit) {
-                    if (k < s) {
-                        atomicOr((unsigned long long*)&adjX[j * WX + (k >> 6)], 1ull << (k & 63));
-                        atomicOr((unsigned long long*)&adjP[k * W + (j >> 6)], 1ull << (j & 63));
-                    } else {
-                        const uint64_t kk = k - s;
-                        atomicOr((unsigned long long*)&adjP[(s + j) * W + (kk >> 6)], 1ull << (kk & 63));
+            const uint64_t s2 = read_lane(incl, kWave - 1);
+            const uint64_t WX2 = (s2 + 63) / 64, FW2 = 3 * W + WX2;
+            const uint64_t rows2 = (s2 + p0) * W + p0 * WX2;
+            const uint64_t nfr = rows2 < kLdsArena ? (kLdsArena - rows2) / FW2 : 0;
+            if (nfr >= kMinLdsFrames) {
+                uint64_t* L = SA;
+                uint64_t* LX = L + (s2 + p0) * W;
+                SL.start[lane] = xmw;  // (the build's work list is no longer needed)
+                SL.pref[lane] = incl - cnt;
+                for (uint64_t i = lane; i < p0 * WX2; i += kWave) LX[i] = 0;
+                for (uint64_t t = 0; t < WX; t++) {  // rows of X' in position order
+                    const uint64_t xm = read_lane(xmw, (uint32_t)t);
+                    if ((xm >> lane) & 1) {
+                        const uint64_t r = read_lane(incl - cnt, (uint32_t)t) + lanes_below(xm), k = t * 64 + lane;
+                        for (uint64_t i = 0; i < W; i++) L[r * W + i] = adjP[k * W + i];
                     }
                 }
+                for (uint64_t i = lane; i < p0 * W; i += kWave) L[s2 * W + i] = adjP[s * W + i];
+                wave_sync();
+                for (uint64_t f = lane; f < p0 * WX; f += kWave) {  // adjX rows over X' positions
+                    const uint64_t j = f / WX, t = f - j * WX;
+                    const uint64_t xm = SL.start[t], before = SL.pref[t];
+                    uint64_t word = adjX[j * WX + t];
+                    while (word) {
+                        const uint32_t b = __builtin_ctzll(word);
+                        word &= word - 1;
+                        const uint64_t c = before + __popcll(xm & ((1ull << b) - 1));
+                        atomicOr((unsigned long long*)&LX[j * WX2 + (c >> 6)], 1ull << (c & 63));
+                    }
+                }
+                aP = L;
+                aX = LX;
+                lfr = LX + p0 * WX2;
+                DL = min<uint64_t>(nfr, p0 + 1);
+                sE = s2;
+                WXE = WX2;
+                FWE = FW2;
+                arena_sync();
             }
-            wave_sync();  // the chunk's work list is rewritten next
         }
-        arena_sync();
+        // frame of depth dep: the first DL in LDS, the rest in the global arena
+        auto frame = [&](uint64_t dep) -> uint64_t* { return dep < DL ? lfr + dep * FWE : frames + dep * FWE; };
 
         // ---- depth-first walk of the root's subtree -----------------------
         uint64_t pw = 0, xpw = 0, qw = 0;
@@ -563,23 +699,25 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
         bool fresh = true;
         if (rec) {
             base = uniform((uint64_t)ld_dev((const unsigned long long*)rec + 1));
-            for (uint64_t i = lane; i < FW; i += kWave) frames[i] = ld_dev((const unsigned long long*)rec + 2 + i);
+            uint64_t* F0 = frame(0);
+            for (uint64_t i = lane; i < FWE; i += kWave) F0[i] = ld_dev((const unsigned long long*)rec + 2 + i);
             arena_sync();
             if (lane < W) {
-                pw = frames[lane];
-                xpw = frames[W + lane];
-                qw = frames[2 * W + lane];
+                pw = F0[lane];
+                xpw = F0[W + lane];
+                qw = F0[2 * W + lane];
             }
             fresh = false;
         } else {
             if (lane < W) pw = ones(p0 - (uint64_t)lane * 64);
-            for (uint64_t i = lane; i < WX; i += kWave) frames[3 * W + i] = ones(s - i * 64);
+            uint64_t* F0 = frame(0);
+            for (uint64_t i = lane; i < WXE; i += kWave) F0[3 * W + i] = ones(sE - i * 64);
         }
         uint64_t depth = 0;  // relative to base
         uint32_t child = 0;  // first-level children passed so far
         uint32_t since = 0;  // nodes since the last look for idle waves
         while (true) {
-            uint64_t* F = frames + depth * FW;
+            uint64_t* F = frame(depth);
             uint64_t* XX = F + 3 * W;
             bool descend = false;
             // the root node belongs to part 0 of a split root
@@ -593,11 +731,11 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                         // hand the upper half of the shallowest ancestor frame's untried
                         // children to an idle wave (not the root frame of a split root)
                         for (uint64_t i = (base == 0 && parts > 1) ? 1 : 0; i < depth; i++) {
-                            uint64_t* Fi = frames + i * FW;
+                            uint64_t* Fi = frame(i);
                             const uint64_t qword = lane < W ? Fi[2 * W + lane] : 0;
                             if (!ballot(qword != 0)) continue;
                             uint64_t idx, off;
-                            if (reserve_record(a, 2 + FW, &idx, &off)) {
+                            if (reserve_record(a, 2 + FWE, &idx, &off)) {
                                 // the donor keeps the lowest half of Q (over all W words)
                                 const uint32_t cnt = __popcll(qword);
                                 uint32_t incl = cnt;
@@ -615,7 +753,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                                     st_dev((unsigned long long*)&a.pool[off + 2 + W + lane], Fi[W + lane] | keepw);
                                     st_dev((unsigned long long*)&a.pool[off + 2 + 2 * W + lane], qword & ~keepw);
                                 }
-                                for (uint64_t k = 3 * W + lane; k < FW; k += kWave)
+                                for (uint64_t k = 3 * W + lane; k < FWE; k += kWave)
                                     st_dev((unsigned long long*)&a.pool[off + 2 + k], Fi[k]);
                                 if (lane < 2)
                                     st_dev((unsigned long long*)&a.pool[off + lane], lane == 0 ? (uint64_t)si : base + i);
@@ -628,12 +766,12 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                 }
                 const bool p_any = ballot(pw != 0) != 0;
                 uint64_t xx_or = 0;
-                for (uint64_t i = lane; i < WX; i += kWave) xx_or |= XX[i];
+                for (uint64_t i = lane; i < WXE; i += kWave) xx_or |= XX[i];
                 const bool x_any = (ballot(xpw != 0) | ballot(xx_or != 0)) != 0;
                 uint64_t np = 0, nPX = 0;
                 if (ACCOUNT) {
                     uint64_t cx = 0;
-                    for (uint64_t i = lane; i < WX; i += kWave) cx += __popcll(XX[i]);
+                    for (uint64_t i = lane; i < WXE; i += kWave) cx += __popcll(XX[i]);
                     np = wave_sum(__popcll(pw));
                     nPX = np + wave_sum(__popcll(xpw)) + wave_sum(cx);
                     if (lane == 0 && own) my_bytes += 5 * (base + depth + 1 + nPX) * (base + depth ? 2 : 1);
@@ -667,7 +805,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                                 const uint64_t pi = read_lane(pw, (uint32_t)i);
 #pragma unroll
                                 for (int q = 0; q < kScoreWords; q++)
-                                    if (on[q]) sc[q] += __popcll(adjP[kq[q] * W + i] & pi);
+                                    if (on[q]) sc[q] += __popcll(aP[kq[q] * W + i] & pi);
                             }
 #pragma unroll
                             for (int q = 0; q < kScoreWords; q++) {
@@ -681,9 +819,9 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                     const uint64_t npw = wave_sum(__popcll(pw));  // |P|
                     // column scoring costs ~|P|/4 rounds of loads, candidate scoring
                     // ~(nonzero X words)/kScoreWords rounds: take the cheaper
-                    const uint64_t xw0 = (WX <= kWave && lane < WX) ? XX[lane] : 0;
+                    const uint64_t xw0 = (WXE <= kWave && lane < WXE) ? XX[lane] : 0;
                     const uint64_t nzx = __popcll(ballot(xw0 != 0));
-                    if (!ACCOUNT && WX <= kWave && npw < 32 && (npw + 3) / 4 < (nzx + kScoreWords - 1) / kScoreWords) {
+                    if (!ACCOUNT && WXE <= kWave && npw < 32 && (npw + 3) / 4 < (nzx + kScoreWords - 1) / kScoreWords) {
                         // Few P members: score all X candidates at once from the
                         // columns of adjX (score(x) = sum over j in P of bit x of
                         // adjX[j], the same number as popc(adjP[x] & P) since the
@@ -700,7 +838,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                                 for (int q = 0; q < 4; q++) {
                                     const uint64_t j = word ? (uint64_t)wi * 64 + __builtin_ctzll(word) : ~0ull;
                                     word &= word ? word - 1 : 0;
-                                    r[q] = (j != ~0ull && lane < WX) ? adjX[j * WX + lane] : 0;
+                                    r[q] = (j != ~0ull && lane < WXE) ? aX[j * WXE + lane] : 0;
                                 }
 #pragma unroll
                                 for (int q = 0; q < 4; q++) {
@@ -729,18 +867,18 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                             best = key > best ? key : best;
                         }
                     } else {
-                        for (uint64_t c0 = 0; c0 < WX; c0 += kWave) {
-                            const uint64_t wx = (c0 + lane < WX) ? XX[c0 + lane] : 0;
+                        for (uint64_t c0 = 0; c0 < WXE; c0 += kWave) {
+                            const uint64_t wx = (c0 + lane < WXE) ? XX[c0 + lane] : 0;
                             score_words(wx, ballot(wx != 0), c0 * 64);
                         }
                     }
                     {
                         const uint64_t uw = pw | xpw;
-                        score_words(uw, ballot(uw != 0), s);
+                        score_words(uw, ballot(uw != 0), sE);
                     }
                     best = wave_max(best);
                     const uint64_t pivot = best & 0xffffffffull;
-                    const uint64_t* prow = adjP + pivot * W;
+                    const uint64_t* prow = aP + pivot * W;
                     qw = lane < W ? (pw & ~prow[lane]) : 0;
                     if (ACCOUNT && own) {
                         if (lane == 0) my_bytes += cost_isect(np, degree_of(a.rp, C[pivot]));
@@ -775,16 +913,16 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                     F[W + lane] = xpw;
                     F[2 * W + lane] = qw;
                 }
-                const uint64_t* qrow = adjP + (s + j) * W;
+                const uint64_t* qrow = aP + (sE + j) * W;
                 if (lane < W) {
                     const uint64_t r = qrow[lane];
                     pw &= r;
                     xpw &= r;
                 }
                 qw = 0;
-                const uint64_t* xrow = adjX + j * WX;
-                uint64_t* XXc = XX + FW;
-                for (uint64_t i = lane; i < WX; i += kWave) XXc[i] = XX[i] & xrow[i];
+                const uint64_t* xrow = aX + j * WXE;
+                uint64_t* XXc = frame(depth + 1) + 3 * W;
+                for (uint64_t i = lane; i < WXE; i += kWave) XXc[i] = XX[i] & xrow[i];
                 depth++;
                 fresh = true;
                 descend = true;
@@ -792,7 +930,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
             if (!descend) {
                 if (depth == 0) break;
                 depth--;
-                const uint64_t* Fp = frames + depth * FW;
+                const uint64_t* Fp = frame(depth);
                 if (lane < W) {
                     pw = Fp[lane];
                     xpw = Fp[W + lane];
