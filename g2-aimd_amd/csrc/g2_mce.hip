@@ -271,21 +271,22 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
     L.adjP[lane] = 0;
     L.adjX[lane] = 0;
     wave_sync();
-    uint32_t len = 0;  // work items of P0 vertex `lane`
-    if (lane < p0) {
-        const uint32_t p = L.C[s + lane];
-        uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
-        if (e1 - e0 > kWave) {  // restrict a long row to [C[0], C[d-1]]
-            e0 = lower_bound(a.cols, e0, e1, L.C[0]);
+    // Oriented build: every edge inside C that touches P0 is found in the
+    // later part of its lower end's row.  Member k = lane (vertex c) streams
+    // N(c) within [v+1, C[d-1]] (c in X0) or [c+1, C[d-1]] (c in P0) and looks
+    // each entry up among C's P0 part; a row of at most 64 entries is streamed
+    // whole (entries outside the range miss, and a P0 pair found from both
+    // ends sets the same bits twice).
+    uint32_t len = 0;
+    if (lane < d) {
+        const uint32_t c = L.C[lane];
+        uint64_t e0 = a.rp[c], e1 = a.rp[c + 1];
+        if (e1 - e0 > kWave) {
+            e0 = lower_bound(a.cols, e0, e1, lane < s ? v + 1 : c + 1);
             e1 = lower_bound(a.cols, e0, e1, L.C[d - 1] + 1);
         }
         L.start[lane] = e0;
-        const uint32_t n = (uint32_t)(e1 - e0);
-        // stream the slice (n items), or binary-search each of the d positions
-        // in it when that is cheaper (hub rows)
-        const bool rev = (uint64_t)d * (1 + bitlen(n)) < n;
-        L.rlen[lane] = rev ? (n | 0x80000000u) : n;
-        len = rev ? d : n;
+        len = (uint32_t)(e1 - e0);
     }
     uint32_t incl = len;
 #pragma unroll
@@ -295,32 +296,30 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
     }
     L.pref[lane] = incl - len;
     const uint32_t T = read_lane(incl, kWave - 1);
+    uint32_t hb_p = 1;  // highest power of two <= p0
+    while (hb_p * 2 <= p0) hb_p *= 2;
     wave_sync();
     for (uint32_t f = lane; f < T; f += kWave) {
-        // P0 vertex j whose streamed slice holds entry f (last j with pref[j] <= f)
-        uint32_t lo2 = 0, hi2 = p0;
+        // member k whose streamed slice holds entry f (last k with pref[k] <= f)
+        uint32_t lo2 = 0, hi2 = d;
         while (hi2 - lo2 > 1) {
             const uint32_t mid = (lo2 + hi2) >> 1;
             if (L.pref[mid] <= f) lo2 = mid; else hi2 = mid;
         }
-        const uint32_t j = lo2;
-        const uint32_t rl = L.rlen[j], item = f - L.pref[j];
-        uint32_t k;
-        bool hit;
-        if (rl & 0x80000000u) {  // position `item` of C looked up in the slice
-            k = item;
-            hit = contains(a.cols, L.start[j], L.start[j] + (rl & 0x7fffffffu), L.C[k]);
-        } else {
-            const uint32_t w = a.cols[L.start[j] + item];
-            k = lds_lower_bound(L.C, 0, d, w);
-            hit = k < d && L.C[k] == w;
-        }
-        if (hit) {
-            if (k < s) {
-                atomicOr((unsigned long long*)&L.adjX[j], 1ull << k);
-                atomicOr((unsigned long long*)&L.adjP[k], 1ull << j);
-            } else {
-                atomicOr((unsigned long long*)&L.adjP[s + j], 1ull << (k - s));
+        const uint32_t k = lo2;
+        const uint32_t w = a.cols[L.start[k] + (f - L.pref[k])];
+        uint32_t pos = s;  // first P0 position with C >= w, fixed halving steps
+        for (uint32_t step = hb_p; step >= 1; step >>= 1)
+            if (pos + step <= d && L.C[pos + step - 1] < w) pos += step;
+        if (pos < d && L.C[pos] == w && pos != k) {
+            const uint32_t j2 = pos - s;
+            if (k < s) {  // X0 member k, P0 member j2
+                atomicOr((unsigned long long*)&L.adjX[j2], 1ull << k);
+                atomicOr((unsigned long long*)&L.adjP[k], 1ull << j2);
+            } else {      // P0 members j1, j2
+                const uint32_t j1 = k - s;
+                atomicOr((unsigned long long*)&L.adjP[s + j1], 1ull << j2);
+                atomicOr((unsigned long long*)&L.adjP[s + j2], 1ull << j1);
             }
         }
     }
@@ -509,22 +508,27 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
         }
         arena_sync();
         if (ldsC) {
-            // Oriented build: every edge inside C that touches P0 is found once,
-            // in the later part of its lower end's row.  Member k (vertex c)
+            // Oriented build: every edge inside C that touches P0 is found in
+            // the later part of its lower end's row.  Member k (vertex c)
             // streams N(c) within [lo, C[d-1]], lo = v+1 for c in X0 (partners
             // in P0) and c+1 for c in P0 (later partners), and looks each entry
             // up among C's P0 part in LDS.  These later-neighbour parts are
-            // short in degeneracy order.
+            // short in degeneracy order.  A row of at most 64 entries is
+            // streamed whole: entries outside the range miss, and a P0 pair
+            // found from both ends sets the same bits twice.
             uint32_t hb_p = 1;  // highest power of two <= p0
             while ((uint64_t)hb_p * 2 <= p0) hb_p *= 2;
             for (uint64_t k0 = 0; k0 < d; k0 += kWave) {
                 const uint32_t nk = (uint32_t)min<uint64_t>(kWave, d - k0);
                 uint32_t len = 0;
-                if (lane < nk) {
+                if (lane < nk) {  // (a row of at most 64 entries is streamed whole)
                     const uint64_t k = k0 + lane;
                     const uint32_t c = Cb[k];
-                    const uint64_t e0 = lower_bound(a.cols, a.rp[c], a.rp[c + 1], k < s ? v + 1 : c + 1);
-                    const uint64_t e1 = lower_bound(a.cols, e0, a.rp[c + 1], Cb[d - 1] + 1);
+                    uint64_t e0 = a.rp[c], e1 = a.rp[c + 1];
+                    if (e1 - e0 > kWave) {
+                        e0 = lower_bound(a.cols, e0, e1, k < s ? v + 1 : c + 1);
+                        e1 = lower_bound(a.cols, e0, e1, Cb[d - 1] + 1);
+                    }
                     SL.start[lane] = e0;
                     len = (uint32_t)(e1 - e0);
                 }
