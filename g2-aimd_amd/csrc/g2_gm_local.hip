@@ -47,19 +47,19 @@
 
 namespace g2 {
 
-// counters: next task, matches, nodes, deferred roots of pass 1 and their
-// largest need, the same for pass 2, largest need that fit no arena
-enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_NB1, LCTR_NEED1, LCTR_NB2, LCTR_NEED2, LCTR_ERR, LCTR_N };
+// counters: next task, matches, nodes, deferred roots of passes 1, 2a and 2b
+// with their largest need, largest need that fit no arena
+enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_NB1, LCTR_NEED1, LCTR_NB2, LCTR_NEED2, LCTR_NB3, LCTR_NEED3, LCTR_ERR, LCTR_N };
 
 constexpr int kLocalBlockWaves = 4;     // gm_local_kernel: __launch_bounds__(256)
 constexpr uint32_t kLRowWords = 448;    // per-wave LDS rows (u64 words)
 constexpr uint32_t kLU = 256;           // per-wave LDS copy of U (u32)
-constexpr int kBWaves = 16;             // gm_local_block_kernel: __launch_bounds__(1024)
-constexpr uint32_t kBRowWords = 17664;  // per-workgroup LDS rows (138 KiB): |U| = 1039 at stride 17
-constexpr uint32_t kBU = 1040;          // per-workgroup |U| limit (at most blockDim.x + 64)
+// workgroup passes: threads, LDS row words, |U| limit, value buckets, and
+// workgroups per CU (the LDS each takes allows no more)
+constexpr int kB1Threads = 256, kB1RowWords = 3136, kB1U = 448, kB1Buckets = 256, kB1PerCu = 4;     // 25 KiB rows: |U| 448 at stride 7
+constexpr int kB2Threads = 1024, kB2RowWords = 17664, kB2U = 1040, kB2Buckets = 512, kB2PerCu = 1;  // 138 KiB rows: |U| 1039 at stride 17
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
-constexpr uint32_t kBuckets = 512;      // value buckets of U in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
 constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
 constexpr uint32_t kFetchRun = 32;      // roots per counter fetch in the first pass
@@ -73,7 +73,7 @@ struct LocalArgs {
     uint64_t* scratch;
     uint64_t scratch_words;
     uint32_t* big_list;                 // deferred roots (null in the last pass)
-    uint32_t defer_ctr;                 // LCTR_NB1 or LCTR_NB2: count of big_list, need next to it
+    uint32_t defer_ctr;                 // LCTR_NB1/2/3: count of big_list, need next to it
     unsigned long long* ctr;
 };
 
@@ -401,24 +401,28 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
     }
 }
 
-// Pass 2: one root per workgroup of 16 waves, its rows in the workgroup's
-// LDS (up to kBRowWords words, |U| <= kBU); roots beyond that are deferred to
-// pass 3.  Waves build 64-member chunks round robin, then take position-1
-// candidates r1 one at a time from an LDS counter; a wave's lanes share the
-// position-2 candidates of its r1, queued 64 at a time in LDS, and each lane
-// walks the rest of its (r1, r2) subtree.
-template <int N>
-__global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
-    __shared__ uint64_t rows[kBRowWords];
-    __shared__ uint32_t U[kBU];
-    __shared__ uint32_t mpref[kBU];   // members' first work item
-    __shared__ uint32_t mstart[kBU];  // members' first entry in ocols
-    __shared__ uint32_t mcode[kBU];   // members' entry count | kRev
-    __shared__ uint32_t queue[kBWaves][kWave];
-    __shared__ uint32_t wsum[kBWaves];
+// Pass 2: one root per workgroup of BT threads, its rows in the workgroup's
+// LDS (up to ROWW words, |U| <= BU); roots beyond that are deferred to the
+// next pass.  Two shapes run in turn: 256 threads with 25 KB of rows (four
+// workgroups per CU, |U| <= 448), then 1024 threads with 138 KB (one per CU).
+// The build is one work list over all members and threads; then waves take
+// position-1 candidates r1 one at a time from an LDS counter, a wave's lanes
+// share the position-2 candidates of its r1, queued 64 at a time in LDS, and
+// each lane walks the rest of its (r1, r2) subtree.
+template <int N, int BT, int ROWW, int BU, int NB>
+__global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
+    constexpr int kWaves = BT / kWave;
+    constexpr int kMembersPerThread = (BU + BT - 1) / BT;
+    __shared__ uint64_t rows[ROWW];
+    __shared__ uint32_t U[BU];
+    __shared__ uint32_t mpref[BU];   // members' first work item
+    __shared__ uint32_t mstart[BU];  // members' first entry in ocols
+    __shared__ uint32_t mcode[BU];   // members' entry count | kRev
+    __shared__ uint32_t queue[kWaves][kWave];
+    __shared__ uint32_t wsum[kWaves];
     __shared__ unsigned long long s_task;
-    __shared__ uint32_t s_next, s_t2, s_maxb;
-    __shared__ uint16_t ubkt[kBuckets + 1];  // first index of U in each value bucket
+    __shared__ uint32_t s_next, s_maxb;
+    __shared__ uint16_t ubkt[NB + 1];  // first index of U in each value bucket
     const uint32_t lane = lane_id();
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t nwaves = blockDim.x / kWave;
@@ -441,7 +445,7 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         const uint32_t W = row_stride(m);
         const uint64_t need = local_need(m);
         if (m < (uint32_t)(N - 1)) continue;
-        if (m > kBU || need > kBRowWords) {
+        if (m > BU || need > ROWW) {
             if (threadIdx.x == 0) defer_root(a, x, need);
             continue;
         }
@@ -451,16 +455,16 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         __syncthreads();
         // value buckets over [U_0, U_{m-1}]: a lookup searches one bucket
         const uint32_t ulo = U[0];
-        const uint64_t bmul = ((uint64_t)kBuckets << 32) / ((uint64_t)U[m - 1] - ulo + 1);
+        const uint64_t bmul = ((uint64_t)NB << 32) / ((uint64_t)U[m - 1] - ulo + 1);
         auto bucket = [&](uint32_t v) { return (uint32_t)(((uint64_t)(v - ulo) * bmul) >> 32); };
         for (uint32_t i = tid; i <= m; i += blockDim.x) {  // buckets (bucket(U_{i-1}), bucket(U_i)] start at i
-            const uint32_t bi = i < m ? bucket(U[i]) : kBuckets;
+            const uint32_t bi = i < m ? bucket(U[i]) : NB;
             for (uint32_t b = i == 0 ? 0 : bucket(U[i - 1]) + 1; b <= bi; b++) ubkt[b] = (uint16_t)i;
         }
         __syncthreads();
         {
             uint32_t big = 0;
-            for (uint32_t b = tid; b < kBuckets; b += blockDim.x) big = max<uint32_t>(big, ubkt[b + 1] - ubkt[b]);
+            for (uint32_t b = tid; b < NB; b += blockDim.x) big = max<uint32_t>(big, ubkt[b + 1] - ubkt[b]);
             big = (uint32_t)wave_max((uint64_t)big);
             if (lane == 0 && big) atomicMax(&s_maxb, big);
         }
@@ -469,11 +473,11 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
         // [U_{i+1}, U_{m-1}]) to look up in U (LDS), or, when n exceeds
         // (m-1-i) x (1 + log2 n), its m-1-i later members to look up in
         // N+(U_i) (global).
-        // Thread t describes members t and t + blockDim.x.
-        uint32_t len[2] = {0, 0};
+        // Thread t describes members t, t + BT, ...; their item counts go to
+        // mpref, which a block scan then turns into first items.
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint32_t i = tid + h * blockDim.x;
+        for (int h = 0; h < kMembersPerThread; h++) {
+            const uint32_t i = tid + h * BT;
             if (i + 1 < m) {
                 const uint32_t u = U[i];
                 uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
@@ -485,37 +489,40 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
                 const bool rev = n > (1u << 20) || rem * (1 + bitlen(n)) < n;
                 mstart[i] = (uint32_t)e0;
                 mcode[i] = rev ? ((uint32_t)n | kRev) : (uint32_t)n;
-                len[h] = rev ? (uint32_t)rem : (uint32_t)n;
+                mpref[i] = rev ? (uint32_t)rem : (uint32_t)n;
+            } else if (i < m) {
+                mpref[i] = 0;
             }
         }
-        // exclusive prefix over member order: members < blockDim.x first
-        uint32_t incl = len[0];
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, kWave);
-            if (lane >= (uint32_t)o) incl += y;
-        }
-        if (lane == kWave - 1) wsum[wid] = incl;
         __syncthreads();
-        uint32_t before = 0, T = 0;
-        for (uint32_t q = 0; q < nwaves; q++) {
-            const uint32_t ws = wsum[q];
-            before += q < wid ? ws : 0;
-            T += ws;
-        }
-        if (tid < m) mpref[tid] = before + incl - len[0];
-        if (m > blockDim.x && wid == 0) {  // members blockDim.x .. m-1 (fewer than 64)
-            uint32_t incl2 = len[1];
+        uint32_t T = 0;
+        {  // exclusive scan of mpref[0..m) in member order, contiguous runs per thread
+            const uint32_t per = (m + BT - 1) / BT;
+            const uint32_t c0 = min(tid * per, m), c1 = min(c0 + per, m);
+            uint32_t sum = 0;
+            for (uint32_t i = c0; i < c1; i++) sum += mpref[i];
+            uint32_t incl = sum;
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t y = __shfl_up(incl2, o, kWave);
-                if (lane >= (uint32_t)o) incl2 += y;
+                const uint32_t y = __shfl_up(incl, o, kWave);
+                if (lane >= (uint32_t)o) incl += y;
             }
-            if (tid + blockDim.x < m) mpref[tid + blockDim.x] = T + incl2 - len[1];
-            if (lane == kWave - 1) s_t2 = incl2;
+            if (lane == kWave - 1) wsum[wid] = incl;
+            __syncthreads();
+            uint32_t before = 0;
+            for (uint32_t q = 0; q < nwaves; q++) {
+                const uint32_t ws = wsum[q];
+                before += q < wid ? ws : 0;
+                T += ws;
+            }
+            uint32_t run = before + incl - sum;
+            for (uint32_t i = c0; i < c1; i++) {
+                const uint32_t l = mpref[i];
+                mpref[i] = run;
+                run += l;
+            }
         }
         __syncthreads();
-        if (m > blockDim.x) T += s_t2;
         const uint32_t nm = m - 1;  // members that own items
         uint32_t hb_own = 1, hb_u = 1;  // highest powers of two <= nm-1 and <= the largest bucket
         while (hb_own * 2 <= nm - 1) hb_own *= 2;
@@ -556,7 +563,7 @@ __global__ __launch_bounds__(1024) void gm_local_block_kernel(LocalArgs a) {
             uint32_t lim[kItems];
 #pragma unroll
             for (int q = 0; q < kItems; q++) {  // (an entry above U_{m-1} lands in the last bucket and misses)
-                const uint32_t b = min(bucket(w[q]), kBuckets - 1);
+                const uint32_t b = min<uint32_t>(bucket(w[q]), NB - 1);
                 pos[q] = ubkt[b];
                 lim[q] = ubkt[b + 1];
             }
@@ -873,20 +880,29 @@ void gm_local_compile(g2_ctx* c, const g2_plan* p) {
     c->local_ok = true;
 }
 
+enum class LocalPass { Wave, Group256, Group1024 };
+
 template <int N>
-static void launch_local(bool block, uint64_t blocks, uint32_t bw, hipStream_t s, const LocalArgs& a) {
+static void launch_local(LocalPass pass, uint64_t blocks, uint32_t bw, hipStream_t s, const LocalArgs& a) {
     if constexpr (N > 2) {
-        if (block) {
-            hipLaunchKernelGGL(gm_local_block_kernel<N>, dim3(blocks), dim3(kBWaves * kWave), 0, s, a);
+        if (pass == LocalPass::Group256) {
+            hipLaunchKernelGGL((gm_local_block_kernel<N, kB1Threads, kB1RowWords, kB1U, kB1Buckets>), dim3(blocks),
+                               dim3(kB1Threads), 0, s, a);
+            return;
+        }
+        if (pass == LocalPass::Group1024) {
+            hipLaunchKernelGGL((gm_local_block_kernel<N, kB2Threads, kB2RowWords, kB2U, kB2Buckets>), dim3(blocks),
+                               dim3(kB2Threads), 0, s, a);
             return;
         }
     }
     hipLaunchKernelGGL(gm_local_kernel<N>, dim3(blocks), dim3(bw * kWave), 0, s, a);
 }
 
-// Three passes over the roots: rows in a wave's LDS; rows in a workgroup's
-// LDS (the roots pass 1 deferred); rows in per-wave global arenas sized to
-// the largest need (the roots pass 2 deferred).
+// Passes over the roots: rows in a wave's LDS; in a 256-thread workgroup's
+// LDS; in a 1024-thread workgroup's LDS; in per-wave global arenas sized to
+// the largest remaining root.  Each pass takes the roots the previous one
+// deferred.
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if (c->nv + 1 > 0x7fffffffull) return set_err(c, G2_EINVAL, "local-bitmap matching supports fewer than 2^31-1 vertices");
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kLocalBlockWaves);
@@ -897,7 +913,9 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if ((rc = ensure(c, c->list, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
-    float kt[3] = {0, 0, 0}, tot = 0;
+    auto* listA = (uint32_t*)c->big_list2.ptr;
+    auto* listB = (uint32_t*)c->list.ptr;
+    float kt[4] = {0, 0, 0, 0}, tot = 0;
     if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
     const uint64_t* orp = nullptr;
     const uint32_t* ocols = nullptr;
@@ -908,27 +926,27 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     a.ocols = ocols;
     a.prog = c->lprog;
     a.ctr = ctr;
-    auto launch = [&](bool block, uint64_t blocks) {
+    auto launch = [&](LocalPass pass, uint64_t blocks) {
         switch (c->lprog.n) {
-            case 2: launch_local<2>(block, blocks, bw, c->stream, a); break;
-            case 3: launch_local<3>(block, blocks, bw, c->stream, a); break;
-            case 4: launch_local<4>(block, blocks, bw, c->stream, a); break;
-            case 5: launch_local<5>(block, blocks, bw, c->stream, a); break;
-            case 6: launch_local<6>(block, blocks, bw, c->stream, a); break;
-            case 7: launch_local<7>(block, blocks, bw, c->stream, a); break;
-            default: launch_local<8>(block, blocks, bw, c->stream, a); break;
+            case 2: launch_local<2>(pass, blocks, bw, c->stream, a); break;
+            case 3: launch_local<3>(pass, blocks, bw, c->stream, a); break;
+            case 4: launch_local<4>(pass, blocks, bw, c->stream, a); break;
+            case 5: launch_local<5>(pass, blocks, bw, c->stream, a); break;
+            case 6: launch_local<6>(pass, blocks, bw, c->stream, a); break;
+            case 7: launch_local<7>(pass, blocks, bw, c->stream, a); break;
+            default: launch_local<8>(pass, blocks, bw, c->stream, a); break;
         }
         return hipGetLastError();
     };
     unsigned long long h[LCTR_N];
-    auto timed = [&](int pass, bool block, uint64_t blocks) -> int {
+    auto timed = [&](int i, LocalPass pass, uint64_t blocks) -> int {
         if ((e = hipMemsetAsync(ctr + LCTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
         if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
-        if ((e = launch(block, blocks))) return hip_err(c, e, "local launch");
+        if ((e = launch(pass, blocks))) return hip_err(c, e, "local launch");
         if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
         if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
         if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
-        (void)hipEventElapsedTime(&kt[pass], c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&kt[i], c->ev[1], c->ev[2]);
         return G2_OK;
     };
     // pass 1: every root, rows in the wave's LDS
@@ -936,33 +954,42 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     a.ntasks = c->nv;
     a.scratch = nullptr;
     a.scratch_words = 0;
-    a.big_list = (uint32_t*)c->big_list2.ptr;
+    a.big_list = listA;
     a.defer_ctr = LCTR_NB1;
-    if ((rc = timed(0, false, nblocks))) return rc;
+    if ((rc = timed(0, LocalPass::Wave, nblocks))) return rc;
     uint32_t launches = 1;
     const uint64_t n1 = h[LCTR_NB1];
-    uint64_t n2 = 0;
-    if (n1) {  // pass 2: one workgroup per CU
-        a.list = (const uint32_t*)c->big_list2.ptr;
+    uint64_t n2 = 0, n3 = 0;
+    if (n1) {  // pass 2a: 256-thread workgroups, four per CU
+        a.list = listA;
         a.ntasks = n1;
-        a.big_list = (uint32_t*)c->list.ptr;
+        a.big_list = listB;
         a.defer_ctr = LCTR_NB2;
-        if ((rc = timed(1, true, std::min<uint64_t>(c->num_cus, n1)))) return rc;
+        if ((rc = timed(1, LocalPass::Group256, std::min<uint64_t>((uint64_t)c->num_cus * kB1PerCu, n1)))) return rc;
         launches++;
         n2 = h[LCTR_NB2];
     }
-    if (n2) {  // pass 3: per-wave global arenas
-        const uint64_t need = h[LCTR_NEED2];
-        uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), n2);
+    if (n2) {  // pass 2b: 1024-thread workgroups, one per CU (list A is free again)
+        a.list = listB;
+        a.ntasks = n2;
+        a.big_list = listA;
+        a.defer_ctr = LCTR_NB3;
+        if ((rc = timed(2, LocalPass::Group1024, std::min<uint64_t>((uint64_t)c->num_cus * kB2PerCu, n2)))) return rc;
+        launches++;
+        n3 = h[LCTR_NB3];
+    }
+    if (n3) {  // pass 3: per-wave global arenas
+        const uint64_t need = h[LCTR_NEED3];
+        uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), n3);
         while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
         bwaves = (bwaves + bw - 1) / bw * bw;
         if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
-        a.list = (const uint32_t*)c->list.ptr;
-        a.ntasks = n2;
+        a.list = listA;
+        a.ntasks = n3;
         a.scratch = (uint64_t*)c->big_scratch.ptr;
         a.scratch_words = need;
         a.big_list = nullptr;
-        if ((rc = timed(2, false, bwaves / bw))) return rc;
+        if ((rc = timed(3, LocalPass::Wave, bwaves / bw))) return rc;
         launches++;
     }
     if ((e = hipEventRecord(c->ev[3], c->stream))) return hip_err(c, e, "event");
@@ -971,13 +998,13 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if (h[LCTR_ERR]) return set_err(c, G2_EBUDGET, "a root did not fit the large-root arena");
     if (count) *count = h[LCTR_COUNT];
     if (st) {
-        st->kernel_ms = (double)kt[0] + (double)kt[1] + (double)kt[2];
+        st->kernel_ms = (double)kt[0] + (double)kt[1] + (double)kt[2] + (double)kt[3];
         st->launches = launches;
         st->total_ms = tot;
         st->nodes = h[LCTR_NODES];
         st->alg_bytes = 0;
         st->big_roots = n1;
-        st->max_scratch_words = n2 ? h[LCTR_NEED2] : h[LCTR_NEED1];
+        st->max_scratch_words = n3 ? h[LCTR_NEED3] : n2 ? h[LCTR_NEED2] : h[LCTR_NEED1];
     }
     return G2_OK;
 }
