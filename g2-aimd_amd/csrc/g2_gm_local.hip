@@ -56,7 +56,7 @@ constexpr uint32_t kLRowWords = 448;    // per-wave LDS rows (u64 words)
 constexpr uint32_t kLU = 256;           // per-wave LDS copy of U (u32)
 // workgroup passes: threads, LDS row words, |U| limit, value buckets, and
 // workgroups per CU (the LDS each takes allows no more)
-constexpr int kB1Threads = 256, kB1RowWords = 3136, kB1U = 448, kB1Buckets = 256, kB1PerCu = 4;     // 25 KiB rows: |U| 448 at stride 7
+constexpr int kB1Threads = 256, kB1RowWords = 3136, kB1U = 448, kB1Buckets = 512, kB1PerCu = 4;     // 25 KiB rows: |U| 448 at stride 7
 constexpr int kB2Threads = 1024, kB2RowWords = 17664, kB2U = 1040, kB2Buckets = 512, kB2PerCu = 1;  // 138 KiB rows: |U| 1039 at stride 17
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
