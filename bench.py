@@ -482,7 +482,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
             "traffic_GBs": traffic / (kernel_ms / 1e3) / 1e9 if traffic else None,
             "traffic_frac": traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
-            "kernel": "gm_local_kernel<4> + gm_local_block_kernel<4> (the two counting passes)",
+            "kernel": "gm_local_kernel<4> + gm_local_block_kernel<4, 256|1024, ...> (the counting passes)",
             "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes,
             "alg_bytes_source": sample_note,
         },
