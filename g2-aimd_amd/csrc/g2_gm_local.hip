@@ -47,9 +47,9 @@
 
 namespace g2 {
 
-// counters: next task, matches, nodes, deferred roots of passes 1, 2a and 2b
-// with their largest need, largest need that fit no arena
-enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_NB1, LCTR_NEED1, LCTR_NB2, LCTR_NEED2, LCTR_NB3, LCTR_NEED3, LCTR_ERR, LCTR_N };
+// counters: next task, matches, nodes, (deferred roots, their largest need)
+// per pass, largest need that fit no arena
+enum { LCTR_NEXT = 0, LCTR_COUNT, LCTR_NODES, LCTR_DEFER, LCTR_ERR = LCTR_DEFER + 8, LCTR_N };
 
 constexpr int kLocalBlockWaves = 4;     // gm_local_kernel: __launch_bounds__(256)
 constexpr uint32_t kLRowWords = 448;    // per-wave LDS rows (u64 words)
@@ -57,7 +57,8 @@ constexpr uint32_t kLU = 256;           // per-wave LDS copy of U (u32)
 // workgroup passes: threads, LDS row words, |U| limit, value buckets, and
 // workgroups per CU (the LDS each takes allows no more)
 constexpr int kB1Threads = 256, kB1RowWords = 3136, kB1U = 448, kB1Buckets = 512, kB1PerCu = 4;     // 25 KiB rows: |U| 448 at stride 7
-constexpr int kB2Threads = 1024, kB2RowWords = 17664, kB2U = 1040, kB2Buckets = 512, kB2PerCu = 1;  // 138 KiB rows: |U| 1039 at stride 17
+constexpr int kB2Threads = 512, kB2RowWords = 7700, kB2U = 700, kB2Buckets = 512, kB2PerCu = 2;      // 60 KiB rows: |U| 700 at stride 11
+constexpr int kB3Threads = 1024, kB3RowWords = 17664, kB3U = 1040, kB3Buckets = 512, kB3PerCu = 1;  // 138 KiB rows: |U| 1039 at stride 17
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
@@ -73,7 +74,7 @@ struct LocalArgs {
     uint64_t* scratch;
     uint64_t scratch_words;
     uint32_t* big_list;                 // deferred roots (null in the last pass)
-    uint32_t defer_ctr;                 // LCTR_NB1/2/3: count of big_list, need next to it
+    uint32_t defer_ctr;                 // LCTR_DEFER + 2 * pass: count of big_list, need next to it
     unsigned long long* ctr;
 };
 
@@ -403,8 +404,9 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
 
 // Pass 2: one root per workgroup of BT threads, its rows in the workgroup's
 // LDS (up to ROWW words, |U| <= BU); roots beyond that are deferred to the
-// next pass.  Two shapes run in turn: 256 threads with 25 KB of rows (four
-// workgroups per CU, |U| <= 448), then 1024 threads with 138 KB (one per CU).
+// next pass.  Three shapes run in turn: 256 threads with 25 KB of rows (four
+// workgroups per CU, |U| <= 448), 512 threads with 60 KB (two per CU, |U| <=
+// 700), then 1024 threads with 138 KB (one per CU).
 // The build is one work list over all members and threads; then waves take
 // position-1 candidates r1 one at a time from an LDS counter, a wave's lanes
 // share the position-2 candidates of its r1, queued 64 at a time in LDS, and
@@ -880,29 +882,35 @@ void gm_local_compile(g2_ctx* c, const g2_plan* p) {
     c->local_ok = true;
 }
 
-enum class LocalPass { Wave, Group256, Group1024 };
+enum class LocalPass { Wave, Group256, Group512, Group1024 };
 
 template <int N>
 static void launch_local(LocalPass pass, uint64_t blocks, uint32_t bw, hipStream_t s, const LocalArgs& a) {
     if constexpr (N > 2) {
-        if (pass == LocalPass::Group256) {
-            hipLaunchKernelGGL((gm_local_block_kernel<N, kB1Threads, kB1RowWords, kB1U, kB1Buckets>), dim3(blocks),
-                               dim3(kB1Threads), 0, s, a);
-            return;
-        }
-        if (pass == LocalPass::Group1024) {
-            hipLaunchKernelGGL((gm_local_block_kernel<N, kB2Threads, kB2RowWords, kB2U, kB2Buckets>), dim3(blocks),
-                               dim3(kB2Threads), 0, s, a);
-            return;
+        switch (pass) {
+            case LocalPass::Group256:
+                hipLaunchKernelGGL((gm_local_block_kernel<N, kB1Threads, kB1RowWords, kB1U, kB1Buckets>), dim3(blocks),
+                                   dim3(kB1Threads), 0, s, a);
+                return;
+            case LocalPass::Group512:
+                hipLaunchKernelGGL((gm_local_block_kernel<N, kB2Threads, kB2RowWords, kB2U, kB2Buckets>), dim3(blocks),
+                                   dim3(kB2Threads), 0, s, a);
+                return;
+            case LocalPass::Group1024:
+                hipLaunchKernelGGL((gm_local_block_kernel<N, kB3Threads, kB3RowWords, kB3U, kB3Buckets>), dim3(blocks),
+                                   dim3(kB3Threads), 0, s, a);
+                return;
+            default:
+                break;
         }
     }
     hipLaunchKernelGGL(gm_local_kernel<N>, dim3(blocks), dim3(bw * kWave), 0, s, a);
 }
 
-// Passes over the roots: rows in a wave's LDS; in a 256-thread workgroup's
-// LDS; in a 1024-thread workgroup's LDS; in per-wave global arenas sized to
-// the largest remaining root.  Each pass takes the roots the previous one
-// deferred.
+// Passes over the roots: rows in a wave's LDS; in a 256-, 512- and
+// 1024-thread workgroup's LDS; in per-wave global arenas sized to the largest
+// remaining root.  Each pass takes the roots the previous one deferred; the
+// deferred lists alternate between two buffers.
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if (c->nv + 1 > 0x7fffffffull) return set_err(c, G2_EINVAL, "local-bitmap matching supports fewer than 2^31-1 vertices");
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kLocalBlockWaves);
@@ -913,9 +921,9 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if ((rc = ensure(c, c->list, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
-    auto* listA = (uint32_t*)c->big_list2.ptr;
-    auto* listB = (uint32_t*)c->list.ptr;
-    float kt[4] = {0, 0, 0, 0}, tot = 0;
+    uint32_t* lists[2] = {(uint32_t*)c->big_list2.ptr, (uint32_t*)c->list.ptr};
+    float tot = 0;
+    double kernel_ms = 0;
     if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
     const uint64_t* orp = nullptr;
     const uint32_t* ocols = nullptr;
@@ -939,14 +947,16 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         return hipGetLastError();
     };
     unsigned long long h[LCTR_N];
-    auto timed = [&](int i, LocalPass pass, uint64_t blocks) -> int {
+    auto timed = [&](LocalPass pass, uint64_t blocks) -> int {
         if ((e = hipMemsetAsync(ctr + LCTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
         if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
         if ((e = launch(pass, blocks))) return hip_err(c, e, "local launch");
         if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
         if ((e = hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H counters");
         if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
-        (void)hipEventElapsedTime(&kt[i], c->ev[1], c->ev[2]);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
+        kernel_ms += ms;
         return G2_OK;
     };
     // pass 1: every root, rows in the wave's LDS
@@ -954,42 +964,37 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     a.ntasks = c->nv;
     a.scratch = nullptr;
     a.scratch_words = 0;
-    a.big_list = listA;
-    a.defer_ctr = LCTR_NB1;
-    if ((rc = timed(0, LocalPass::Wave, nblocks))) return rc;
+    a.big_list = lists[0];
+    a.defer_ctr = LCTR_DEFER;
+    if ((rc = timed(LocalPass::Wave, nblocks))) return rc;
     uint32_t launches = 1;
-    const uint64_t n1 = h[LCTR_NB1];
-    uint64_t n2 = 0, n3 = 0;
-    if (n1) {  // pass 2a: 256-thread workgroups, four per CU
-        a.list = listA;
-        a.ntasks = n1;
-        a.big_list = listB;
-        a.defer_ctr = LCTR_NB2;
-        if ((rc = timed(1, LocalPass::Group256, std::min<uint64_t>((uint64_t)c->num_cus * kB1PerCu, n1)))) return rc;
+    const uint64_t n1 = h[LCTR_DEFER];
+    uint64_t n = n1, need = h[LCTR_DEFER + 1];
+    int in = 0;  // list holding the roots of the next pass
+    const struct { LocalPass pass; int per_cu; } groups[3] = {
+        {LocalPass::Group256, kB1PerCu}, {LocalPass::Group512, kB2PerCu}, {LocalPass::Group1024, kB3PerCu}};
+    for (int g = 0; g < 3 && n; g++) {  // workgroup passes
+        a.list = lists[in];
+        a.ntasks = n;
+        a.big_list = lists[in ^ 1];
+        a.defer_ctr = LCTR_DEFER + 2 * (g + 1);
+        if ((rc = timed(groups[g].pass, std::min<uint64_t>((uint64_t)c->num_cus * groups[g].per_cu, n)))) return rc;
         launches++;
-        n2 = h[LCTR_NB2];
+        n = h[a.defer_ctr];
+        need = h[a.defer_ctr + 1];
+        in ^= 1;
     }
-    if (n2) {  // pass 2b: 1024-thread workgroups, one per CU (list A is free again)
-        a.list = listB;
-        a.ntasks = n2;
-        a.big_list = listA;
-        a.defer_ctr = LCTR_NB3;
-        if ((rc = timed(2, LocalPass::Group1024, std::min<uint64_t>((uint64_t)c->num_cus * kB2PerCu, n2)))) return rc;
-        launches++;
-        n3 = h[LCTR_NB3];
-    }
-    if (n3) {  // pass 3: per-wave global arenas
-        const uint64_t need = h[LCTR_NEED3];
-        uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), n3);
+    if (n) {  // last pass: per-wave global arenas
+        uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), n);
         while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
         bwaves = (bwaves + bw - 1) / bw * bw;
         if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
-        a.list = listA;
-        a.ntasks = n3;
+        a.list = lists[in];
+        a.ntasks = n;
         a.scratch = (uint64_t*)c->big_scratch.ptr;
         a.scratch_words = need;
         a.big_list = nullptr;
-        if ((rc = timed(3, LocalPass::Wave, bwaves / bw))) return rc;
+        if ((rc = timed(LocalPass::Wave, bwaves / bw))) return rc;
         launches++;
     }
     if ((e = hipEventRecord(c->ev[3], c->stream))) return hip_err(c, e, "event");
@@ -998,13 +1003,13 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if (h[LCTR_ERR]) return set_err(c, G2_EBUDGET, "a root did not fit the large-root arena");
     if (count) *count = h[LCTR_COUNT];
     if (st) {
-        st->kernel_ms = (double)kt[0] + (double)kt[1] + (double)kt[2] + (double)kt[3];
+        st->kernel_ms = kernel_ms;
         st->launches = launches;
         st->total_ms = tot;
         st->nodes = h[LCTR_NODES];
         st->alg_bytes = 0;
         st->big_roots = n1;
-        st->max_scratch_words = n3 ? h[LCTR_NEED3] : n2 ? h[LCTR_NEED2] : h[LCTR_NEED1];
+        st->max_scratch_words = need;
     }
     return G2_OK;
 }
