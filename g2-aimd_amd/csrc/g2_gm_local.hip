@@ -23,20 +23,21 @@
 // instead of their full rows (a hub sits high in the order, so its N+ is
 // short while its full row is the longest).
 //
-// Per root x (one wave64 each, dynamic fetch):
+// Per root x:
 //   1. U = N+(x), m = |U|, bit index = position in U;
 //   2. rows[i] = N(U_i) & U as m-bit sets, from N+(U_i) & U[i+1..m) (both
-//      bits set per found pair), by one flattened work list per 64 members
+//      bits set per found pair), by one flattened work list over the members
 //      (stream N+(U_i) against U, or look U up in N+(U_i) when that row is
 //      long);
-//   3. each lane walks the matches of one position-1 candidate depth first:
+//   3. the matches are walked depth first from the position-1 candidates:
 //      cand(i) = AND of the rows of its back neighbours & order ranges &
 //      NON_EQUAL holes, recomputed word by word (no stacks); the last
 //      position is a popcount over the words its order range leaves.
-// Rows of roots with m <= 256 and m * ceil(m/64) <= 448 words live in LDS
-// (4.5 KB per wave, 8 blocks of 4 waves per CU); larger roots use a global
-// per-wave arena, and roots beyond that are deferred to a second launch
-// with larger arenas.
+// The roots go through passes, each taking the roots the previous one
+// deferred: one wave per root with the rows in the wave's LDS (|U| <= 149);
+// one workgroup of 256, 512 or 1024 threads per root with the rows in the
+// workgroup's LDS (|U| <= 448, 700, 1039); one wave per root with the rows in
+// a global arena sized to the largest remaining root.
 #include <algorithm>
 #include <vector>
 
