@@ -127,11 +127,12 @@ def test_local_path_large_root_pass(g2, orc):
 
 
 def test_local_path_complete_graph(g2, gpu_exec):
-    """K_170: every root's U is the rest of the order (|U| up to 169), and the
-    k-clique totals are the binomials C(170, k)."""
+    """K_170: every root's U is the rest of the order (|U| up to 169, so the
+    wave and 256-thread workgroup passes both run), and the k-clique totals
+    are the binomials C(170, k), up to 6-cliques."""
     print("case: local path complete graph", flush=True)
     gpu_exec.transfer(G.complete(g2, 170))
-    for q, k in ((0, 3), (24, 4), (25, 5)):
+    for q, k in ((0, 3), (24, 4), (25, 5), (26, 6)):
         gpu_exec.set_plan(g2.plan_for_preset(q))
         assert gpu_exec.run_gm()["count"] == math.comb(170, k)
 
