@@ -17,9 +17,14 @@
 //   * the root's neighbourhood C = N(v) (sorted, position order == id order)
 //     becomes bitsets: adjP[k] = N(C[k]) & P0 over the p0 = |P0| later
 //     neighbours, and adjX[j] = N(P0[j]) & X0 over the s earlier ones.  Set
-//     intersection is then AND + popcount on 64-bit words;
+//     intersection is then AND + popcount on 64-bit words.  The bitsets are
+//     built from the later parts of the members' rows (each edge inside C
+//     found at its lower end), short in degeneracy order;
 //   * a node is (P, XP) in P0 space (one 64-bit word per lane, lane i holds
-//     word i) plus XX in X0 space (a stack frame in the wave's HBM arena);
+//     word i) plus XX in X0 space (a stack frame).  Roots of degree <= 64 keep
+//     everything in registers; larger ones keep the rows over X' (the X0
+//     members with a P0 neighbour) and the shallow frames in the wave's LDS
+//     when they fit, else in the wave's global arena;
 //   * pivot scoring gives each candidate to one lane and reduces the key
 //     (score << 32 | position) with a 64-bit wave max: max score, then the
 //     largest position == the reference's "last maximum" rule.
