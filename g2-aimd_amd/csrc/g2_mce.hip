@@ -1046,19 +1046,19 @@ __global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
         const uint32_t gmask = G == 32 ? 0xffffffffu : 0xffffu;
         const uint32_t smask = (uint32_t)(ballot(gl < d && L.C[lane] < v) >> gbase) & gmask;
         const uint32_t s = __popc(smask), p0 = d - s;
+        // oriented build, as in the register path: member gl (vertex c)
+        // streams the later part of its row and looks entries up among the
+        // group's P0 part; a row of at most 64 entries is streamed whole
         uint32_t len = 0;
-        if (gl < p0) {
-            const uint32_t p = L.C[gbase + s + gl];
-            uint64_t e0 = a.rp[p], e1 = a.rp[p + 1];
+        if (gl < d) {
+            const uint32_t c = L.C[lane];
+            uint64_t e0 = a.rp[c], e1 = a.rp[c + 1];
             if (e1 - e0 > kWave) {
-                e0 = lower_bound(a.cols, e0, e1, L.C[gbase]);
+                e0 = lower_bound(a.cols, e0, e1, gl < s ? v + 1 : c + 1);
                 e1 = lower_bound(a.cols, e0, e1, L.C[gbase + d - 1] + 1);
             }
             L.start[lane] = e0;
-            const uint32_t n = (uint32_t)(e1 - e0);
-            const bool rev = (uint64_t)d * (1 + bitlen(n)) < n;
-            L.rlen[lane] = rev ? (n | 0x80000000u) : n;
-            len = rev ? d : n;
+            len = (uint32_t)(e1 - e0);
         }
         uint32_t incl = len;
 #pragma unroll
@@ -1070,29 +1070,23 @@ __global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
         const uint32_t T = group_read<G>(incl, gbase, G - 1);
         wave_sync();
         for (uint32_t f = gl; f < T; f += G) {
-            uint32_t lo2 = 0, hi2 = p0;
+            uint32_t lo2 = 0, hi2 = d;
             while (hi2 - lo2 > 1) {
                 const uint32_t mid = (lo2 + hi2) >> 1;
                 if (L.pref[gbase + mid] <= f) lo2 = mid; else hi2 = mid;
             }
-            const uint32_t j = lo2;
-            const uint32_t rl = L.rlen[gbase + j], item = f - L.pref[gbase + j];
-            uint32_t k;
-            bool hit;
-            if (rl & 0x80000000u) {
-                k = item;
-                hit = contains(a.cols, L.start[gbase + j], L.start[gbase + j] + (rl & 0x7fffffffu), L.C[gbase + k]);
-            } else {
-                const uint32_t w = a.cols[L.start[gbase + j] + item];
-                k = lds_lower_bound(L.C + gbase, 0, d, w);
-                hit = k < d && L.C[gbase + k] == w;
-            }
-            if (hit) {
-                if (k < s) {
-                    atomicOr((unsigned long long*)&L.adjX[gbase + j], 1ull << k);
-                    atomicOr((unsigned long long*)&L.adjP[gbase + k], 1ull << j);
-                } else {
-                    atomicOr((unsigned long long*)&L.adjP[gbase + s + j], 1ull << (k - s));
+            const uint32_t k = lo2;
+            const uint32_t w = a.cols[L.start[gbase + k] + (f - L.pref[gbase + k])];
+            const uint32_t pos = lds_lower_bound(L.C + gbase, s, d, w);
+            if (pos < d && L.C[gbase + pos] == w && pos != k) {
+                const uint32_t j2 = pos - s;
+                if (k < s) {  // X0 member k, P0 member j2
+                    atomicOr((unsigned long long*)&L.adjX[gbase + j2], 1ull << k);
+                    atomicOr((unsigned long long*)&L.adjP[gbase + k], 1ull << j2);
+                } else {      // P0 members j1, j2
+                    const uint32_t j1 = k - s;
+                    atomicOr((unsigned long long*)&L.adjP[gbase + s + j1], 1ull << j2);
+                    atomicOr((unsigned long long*)&L.adjP[gbase + s + j2], 1ull << j1);
                 }
             }
         }
