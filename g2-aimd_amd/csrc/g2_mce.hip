@@ -266,6 +266,10 @@ __device__ inline uint32_t wave_max32(uint32_t x) {
 // With rec != nullptr the walk resumes a donated frame (W = WX = 1 here):
 // it starts at that node with its untried children, one level above which
 // the walk ends.
+__device__ inline uint64_t small_walk(const MceArgs& a, uint32_t si, uint32_t s, uint32_t d, uint64_t rowP, uint64_t rowX,
+                                      uint32_t part, uint32_t parts, const uint64_t* rec, uint64_t& cliques,
+                                      uint64_t& hcount, uint64_t& hbig);
+
 __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si, uint32_t v, uint64_t lo, uint32_t d,
                                       uint32_t s, uint32_t part, uint32_t parts, const uint64_t* rec,
                                       uint64_t& cliques, uint64_t& hcount, uint64_t& hbig) {
@@ -329,9 +333,18 @@ __device__ inline uint64_t small_root(const MceArgs& a, SmallLds& L, uint32_t si
         }
     }
     wave_sync();
-    const uint64_t rowP = L.adjP[lane];
-    const uint64_t rowX = L.adjX[lane];
+    return small_walk(a, si, s, d, L.adjP[lane], L.adjX[lane], part, parts, rec, cliques, hcount, hbig);
+}
 
+// The register walk of a root whose rows are in registers: lane k holds row
+// rowP (position k over P0) and, for k < p0, rowX (P0 member k over X).  Also
+// used by larger roots whose rows over X' (the X0 members with a neighbour in
+// P0) and P0 fit one word each and |X'| + p0 <= 64.
+__device__ inline uint64_t small_walk(const MceArgs& a, uint32_t si, uint32_t s, uint32_t d, uint64_t rowP, uint64_t rowX,
+                                      uint32_t part, uint32_t parts, const uint64_t* rec, uint64_t& cliques,
+                                      uint64_t& hcount, uint64_t& hbig) {
+    const uint32_t lane = lane_id();
+    const uint32_t p0 = d - s;
     // ---- depth-first walk (depth relative to `base`, the start node's depth) ----
     uint64_t P = ones(p0), XP = 0, XX = ones(s), Q = 0;
     uint32_t base = 0;
@@ -698,6 +711,17 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                 FWE = FW2;
                 arena_sync();
             }
+        }
+        // rows over X' and P0 of one word each, |X'| + p0 <= 64: the register walk
+        if (!ACCOUNT && DL > 0 && W == 1 && WXE == 1 && sE + p0 <= kWave) {
+            const uint64_t rowP = lane < sE + p0 ? aP[lane] : 0;
+            const uint64_t rowX = lane < p0 ? aX[lane] : 0;
+            root_nodes = small_walk(a, si, (uint32_t)sE, (uint32_t)(sE + p0), rowP, rowX, part, parts, rec, small_cliques,
+                                    small_hist, small_big);
+            my_nodes += root_nodes;
+            if (a.per_root && lane == 0 && root_nodes) atomicAdd(&a.per_root[si], (unsigned long long)root_nodes);
+            t_end = wall_clock64();
+            return;
         }
         // frame of depth dep: the first DL in LDS, the rest in the global arena
         auto frame = [&](uint64_t dep) -> uint64_t* { return dep < DL ? lfr + dep * FWE : frames + dep * FWE; };
