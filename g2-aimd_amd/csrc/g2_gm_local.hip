@@ -64,7 +64,7 @@ constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later member
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
 constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
-constexpr uint32_t kFetchRun = 32;      // roots per counter fetch in the first pass
+constexpr uint32_t kFetchRun = 8;       // roots per counter fetch in the first pass (a sweep of 4..64 favoured 8)
 
 struct LocalArgs {
     const uint64_t* __restrict__ orp;   // oriented rows over ranks
