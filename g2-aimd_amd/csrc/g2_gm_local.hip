@@ -147,7 +147,7 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
                                             uint32_t W, uint32_t a0) {
     const uint32_t lane = lane_id();
     const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
-    uint64_t st = 0;
+    uint32_t st = 0;  // (row offsets fit 32 bits: this path runs with fewer than 2^32 row entries)
     uint32_t code = 0, len = 0;
     if (lane < na) {
         const uint32_t ia = a0 + lane, u = U[ia];
@@ -164,7 +164,7 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
         }
         const uint64_t n = e1 - e0;
         rev = rev || n > (1u << 20);
-        st = e0;
+        st = (uint32_t)e0;
         code = rev ? ((uint32_t)n | kRev) : (uint32_t)n;
         len = rev ? (uint32_t)rem : (uint32_t)n;
     }
@@ -176,12 +176,11 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
     }
     const uint32_t pref = incl - len;
     const uint32_t T = read_lane(incl, kWave - 1);
-    const uint32_t st_lo = (uint32_t)st, st_hi = (uint32_t)(st >> 32);
     uint32_t hb_u = 1;  // highest power of two <= m
     while (hb_u * 2 <= m) hb_u *= 2;
     for (uint32_t f0 = 0; f0 < T; f0 += kItemsW * kWave) {
         uint32_t f[kItemsW], j[kItemsW], item[kItemsW], cj[kItemsW], w[kItemsW], pos[kItemsW];
-        uint64_t sj[kItemsW];
+        uint32_t sj[kItemsW];
 #pragma unroll
         for (int q = 0; q < kItemsW; q++) {
             f[q] = f0 + q * kWave + lane;
@@ -203,13 +202,13 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
         for (int q = 0; q < kItemsW; q++) {
             item[q] = f[q] - __shfl(pref, (int)j[q], kWave);
             cj[q] = __shfl(code, (int)j[q], kWave);
-            sj[q] = ((uint64_t)__shfl(st_hi, (int)j[q], kWave) << 32) | __shfl(st_lo, (int)j[q], kWave);
+            sj[q] = __shfl(st, (int)j[q], kWave);
             const bool valid = f[q] < T;
             if (valid && (cj[q] & kRev)) {
                 w[q] = U[a0 + j[q] + 1 + item[q]];  // the later member to look up
                 hb_r = max(hb_r, cj[q] & ~kRev);
             } else {
-                w[q] = valid ? a.ocols[sj[q] + item[q]] : 0;
+                w[q] = valid ? a.ocols[(uint64_t)sj[q] + item[q]] : 0;
             }
             pos[q] = 0;
         }
@@ -235,7 +234,7 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             for (int q = 0; q < kItemsW; q++) {
                 const uint32_t c = rpos[q] + step;
                 if ((cj[q] & kRev) && c <= (cj[q] & ~kRev)) {
-                    const uint32_t v = a.ocols[sj[q] + c - 1];
+                    const uint32_t v = a.ocols[(uint64_t)sj[q] + c - 1];
                     if (v <= w[q]) {
                         rpos[q] = c;
                         last[q] = v;
