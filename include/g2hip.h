@@ -61,12 +61,14 @@ typedef struct g2_tuning {
 
 /* measurements of the last run */
 typedef struct g2_stats {
-    double   kernel_ms;           /* HIP-event time of the dominant search kernel, all launches */
+    double   kernel_ms;           /* HIP-event time of the search kernels, all launches (local-bitmap
+                                     matching: its counting passes, not the oriented-row build) */
     uint32_t launches;            /* launches of the dominant search kernel */
     double   total_ms;            /* HIP-event time of the whole run on the context's stream */
     uint64_t nodes;               /* MCE search-tree nodes / GM partial-match extensions */
     uint64_t alg_bytes;           /* modelled algorithmic bytes (only with G2_RUN_ACCOUNT) */
-    uint64_t big_roots;           /* roots routed to the large-root pass */
+    uint64_t big_roots;           /* roots routed to the large-root pass (MCE, canonical matching), or
+                                     deferred by the first local-bitmap pass to the workgroup passes */
     uint64_t max_scratch_words;   /* largest per-root scratch need seen */
     double   wave_idle_frac;      /* MCE: 1 - sum of per-wave busy spans / (waves x makespan),
                                      from the GPU real-time clock (tail idleness) */
