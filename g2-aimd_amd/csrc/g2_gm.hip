@@ -550,9 +550,10 @@ int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_st
     const bool account = flags & G2_RUN_ACCOUNT;
     // whole-graph totals of a root-dominated plan: oriented local bitmaps
     // (per-root counts and the byte model follow the reference's traversal)
-    // (its workgroup pass keeps row offsets in 32 bits: fewer than 2^32 row entries)
-    if (c->local_ok && c->sources_all && c->ne < (1ull << 32) && !per_root && !account &&
-        !(flags & G2_RUN_CANONICAL))
+    // (it keeps row offsets in 32 bits and sorts ranks as int-sized lists: fewer
+    // than 2^32 row entries and 2^31 - 1 vertices)
+    if (c->local_ok && c->sources_all && c->ne < (1ull << 32) && c->nv + 1 <= 0x7fffffffull && !per_root &&
+        !account && !(flags & G2_RUN_CANONICAL))
         return gm_local_run(c, count, st);
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM) / bw);
