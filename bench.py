@@ -4,8 +4,8 @@
 Workload (N=1): 4-clique matching (preset q=24, the reference's `run -q 24`)
 over every root of a seeded R-MAT graph, scale 22, edge factor 16,
 (a,b,c,d) = (.57,.19,.19,.05), symmetrised, deduplicated, self loops and
-degree-0 vertices removed -- the graph of BASELINE config "R-MAT scale 22"
-resident in HBM as one view.  One step = one full g2_gm_run over all roots.
+degree-0 vertices removed -- the R-MAT scale-22 matching workload, resident
+in HBM as one view.  One step = one full g2_gm_run over all roots.
 Also reported: triangles (q=0) on the same graph and maximal cliques on the
 Chung-Lu n=100k graph (secondary lines inside the JSON object).
 
@@ -433,8 +433,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
-              "the hot path shards by root ranges with one host-summed u64 per GPU, "
-              "see DESIGN.md", file=sys.stderr)
+              "the hot path would shard by root ranges, with one u64 count (and the clique "
+              "histogram) summed over GPUs", file=sys.stderr)
         sys.exit(2)
 
     g2 = importlib.import_module("g2-aimd_amd")
