@@ -1,4 +1,4 @@
-// oracle/ref_plan_dump.cpp -- driver for the reference's own query-plan code.
+// oracle/ref_plan_dump.cpp -- test program around the reference's own query-plan code.
 //
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) against
 // the reference's headers where they lie (common/graph.h, system/plan.h) and
