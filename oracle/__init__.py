@@ -70,6 +70,14 @@ def lib() -> C.CDLL:
     path = os.path.join(_HERE, "liboracle.so")
     if not os.path.exists(path):
         build()
+    _lib = bind(path)
+    return _lib
+
+
+def bind(path: str) -> C.CDLL:
+    """Load an oracle library built from oracle.cpp and declare its signatures.
+    The tests also use it for copies of the oracle with a planted mistake, to
+    show that their checks catch it."""
     L = C.CDLL(path)
     P64 = C.POINTER(C.c_uint64)
     L.orc_mce.argtypes = [u64p, C.c_uint64, u32p, C.c_uint64, u32p, C.c_uint64, C.c_int,
@@ -81,7 +89,6 @@ def lib() -> C.CDLL:
     L.orc_automorphism_count.argtypes = [C.c_uint32, u32p]
     L.orc_automorphism_count.restype = C.c_uint64
     L.orc_kclique_count.argtypes = [u64p, C.c_uint64, u32p, C.c_uint32, C.c_int, P64]
-    _lib = L
     return L
 
 

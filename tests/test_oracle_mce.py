@@ -87,3 +87,32 @@ def test_per_root_nodes_tiny(g2, orc):
     r = orc.mce(g, per_root=True)
     assert r["total"] == 2
     assert list(r["per_root_nodes"]) == [2, 2, 1]
+
+
+def test_per_root_nodes_tie_rule(g2, orc):
+    # Pins the pivot rule of app_BK/BK.h:282-321: the pivot ranges over P and X,
+    # and ties go to the LAST maximum in list order (ids ascending).
+    #
+    # Triangle 0-1-2, root 1: R={1}, P={2}, X={0}.  Scores |P & N(u)|: 0 -> 1,
+    # 2 -> 0, so the pivot is the X member 0 and Q = P \ N(0) = {} : 1 node.
+    # A pivot taken from P alone would be 2, with one more child: 2 nodes.
+    tri = G.from_pairs(g2, [(0, 1), (0, 2), (1, 2)], 3)
+    assert list(orc.mce(tri, per_root=True)["per_root_nodes"]) == [3, 1, 1]
+
+    # Vertex 0 joined to 1..5, with 1-2, 1-3, 2-3, 2-4, 3-4, 4-5 among them.
+    # Root 0: P={1..5}.  Scores 1:2, 2:3, 3:3, 4:3, 5:1 -> last maximum 4,
+    # Q = P \ N(4) = {1,4}.
+    #   q=1: P={2,3}; scores 1,1 -> pivot 3, Q={3}; child R+3, P={2};
+    #        pivot 2, Q={2}; child {0,1,3,2} is a clique.            3 nodes
+    #   q=4: P={2,3,5} (1 is not adjacent to 4); scores 1,1,0 -> pivot 3,
+    #        Q={3,5}; q=3: P={2} -> pivot 2 -> clique {0,4,3,2}: 2 nodes;
+    #        q=5: nothing adjacent -> clique {0,4,5}: 1 node.          4 nodes
+    #   root total 1 + 3 + 4 = 8.  (Pivot 2, the first maximum, gives 7.)
+    # Roots 1..5 each have 0 in X, adjacent to all of P.  Its score |P| is the
+    # maximum, and any member tying with it is also adjacent to all of P, so
+    # Q is empty: 1 node each.
+    g = G.from_pairs(g2, [(0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (1, 2), (1, 3), (2, 3),
+                          (2, 4), (3, 4), (4, 5)], 6)
+    r = orc.mce(g, per_root=True)
+    assert r["total"] == 3 and hist_dict(r["hist"]) == {3: 1, 4: 2}
+    assert list(r["per_root_nodes"]) == [8, 1, 1, 1, 1, 1]

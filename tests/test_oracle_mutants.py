@@ -1,0 +1,171 @@
+"""Shows that the checks pinning the oracle are not vacuous.
+
+Each case below plants one plausible mistake in a copy of oracle/oracle.cpp
+(a dropped term, a wrong label, an inverted test, an off-by-one), builds that
+copy on the CPU, and runs the pinning tests of test_oracle_mce.py or
+test_oracle_gm.py against it.  At least one of them must fail.  The mistakes
+are planted only in the restatements (orc_mce, orc_gm and their helpers), never
+in the independent checkers they are compared with.
+"""
+import json
+import os
+import subprocess
+
+import pytest
+
+import test_oracle_gm as TG
+import test_oracle_mce as TM
+
+ORACLE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "oracle", "oracle.cpp")
+CHECKERS_MARK = "// Independent checkers (brute force)"
+
+# (name, suite, original text, planted text)
+MUTANTS = [
+    # app_BK/BK.h:78-91: a node is a maximal clique only when P and X are empty
+    ("mce_clique_test_ignores_x", "mce",
+     "if (e.l == LP || e.l == LX) { anyPX = true; break; }",
+     "if (e.l == LP) { anyPX = true; break; }"),
+    # app_BK/BK.h:282-321: ties go to the last maximum
+    ("mce_pivot_first_maximum", "mce",
+     "if (cnt >= maxc) { maxc = cnt; pivot = u; }",
+     "if (cnt > maxc || nPX == 1) { maxc = cnt; pivot = u; }"),
+    # app_BK/BK.h:282-321: the pivot ranges over P and X
+    ("mce_pivot_from_p_only", "mce",
+     "        if (e.l == LR) continue;\n        nPX++;",
+     "        if (e.l != LP) continue;\n        nPX++;"),
+    # app_BK/BK.h:445-464: Q = P minus N(pivot)
+    ("mce_q_inverted", "mce",
+     "if (e.l == LP && !member(g.cols, plo, phi, e.v)) e.l = LQ;",
+     "if (e.l == LP && member(g.cols, plo, phi, e.v)) e.l = LQ;"),
+    # app_BK/BK.h:236-280: earlier Q siblings move to X in the child
+    ("mce_earlier_siblings_stay_in_p", "mce",
+     "if (l == LQ) l = (e.v < q) ? LX : LP;",
+     "if (l == LQ) l = LP;"),
+    # app_BK/BK.h:323-355: neighbours below the root start in X
+    ("mce_root_without_x", "mce",
+     "root.push_back({u, u < v ? LX : LP});",
+     "root.push_back({u, LP});"),
+    # histogram bin = clique size
+    ("mce_histogram_off_by_one", "mce",
+     "uint64_t b = std::min<uint64_t>(nR, acc.hist.size() - 1);",
+     "uint64_t b = std::min<uint64_t>(nR - 1, acc.hist.size() - 1);"),
+    # app_BK/BK.h:335-336: degree-0 roots are skipped
+    ("mce_isolated_roots_counted", "mce",
+     "if (hi > lo) {  // deg-0 roots",
+     "if (true) {  // deg-0 roots"),
+    # app_gmatch/gmatch.h:559-588: NON_EQUAL conditions
+    ("gm_non_equal_ignored", "gm",
+     "else { if (m == w) return false; }",
+     "else { }"),
+    # app_gmatch/gmatch.h:559-588: LARGER is strict
+    ("gm_larger_not_strict", "gm",
+     "else if (op == C_LARGER) { if (m >= w) return false; }",
+     "else if (op == C_LARGER) { if (m > w) return false; }"),
+    # app_gmatch/gmatch.h:589-601: membership in every other back neighbour
+    ("gm_membership_skipped", "gm",
+     "            const uint32_t bm = c.M[c.p->rev[bns[j]]];\n"
+     "            ok = member(c.g.cols, c.g.rp[bm], c.g.rp[bm + 1], w);",
+     "            const uint32_t bm = c.M[c.p->rev[bns[j]]];\n"
+     "            ok = bm == bm;"),
+    # app_gmatch/gmatch.h:273-509: shared positions filter by the after set
+    ("gm_shared_after_membership_skipped", "gm",
+     "                const uint32_t bm = c.M[p->rev[p->after_bn[id][j]]];\n"
+     "                ok = member(c.g.cols, c.g.rp[bm], c.g.rp[bm + 1], w);",
+     "                const uint32_t bm = c.M[p->rev[p->after_bn[id][j]]];\n"
+     "                ok = bm == bm;"),
+    ("gm_shared_after_conditions_skipped", "gm",
+     "if (!conds_ok(c, p->after_cond_count[u], p->after_cond[u], w)) continue;",
+     "(void)u;"),
+    ("gm_shared_pre_conditions_skipped", "gm",
+     "p->pre_cond_count[u], p->pre_cond[u], pre);",
+     "0, p->pre_cond[u], pre);"),
+    # the last position adds every candidate, not one per prefix
+    ("gm_last_level_counts_prefixes", "gm",
+     "if (id + 1 == p->n) { c.count += cands.size(); return; }",
+     "if (id + 1 == p->n) { c.count += !cands.empty(); return; }"),
+]
+
+MCE_PINS = ["test_bruteforce", "test_complete_graph", "test_complete_bipartite", "test_cycle",
+            "test_moon_moser", "test_turan", "test_disjoint_union_and_isolated",
+            "test_relabel_invariance_and_root_partition", "test_per_root_nodes_tiny",
+            "test_per_root_nodes_tie_rule"]
+GM_PINS = ["test_cliques_closed_form", "test_presets_vs_bruteforce",
+           "test_random_patterns_vs_bruteforce", "test_kclique_independent_method",
+           "test_per_root_counts_sum"]
+
+
+def pin_cases(mod, names):
+    """Every (label, function, kwargs) case of the named tests, parametrized
+    ones expanded from their own parametrize marks."""
+    for name in names:
+        fn = getattr(mod, name)
+        marks = [m for m in getattr(fn, "pytestmark", []) if m.name == "parametrize"]
+        assert len(marks) <= 1, name
+        if not marks:
+            yield name, fn, {}
+            continue
+        keys = [k.strip() for k in marks[0].args[0].split(",")]
+        for vals in marks[0].args[1]:
+            vals = vals if isinstance(vals, tuple) else (vals,)
+            yield f"{name}{vals}", fn, dict(zip(keys, vals))
+
+
+@pytest.fixture(scope="module")
+def mutant_libs(tmp_path_factory):
+    with open(ORACLE_SRC) as f:
+        src = f.read()
+    cut = src.index(CHECKERS_MARK)
+    out = tmp_path_factory.mktemp("oracle_mutants")
+    procs = {}
+    for name, _, old, new in MUTANTS:
+        assert src.count(old) == 1, f"{name}: the original text must occur exactly once"
+        assert src.index(old) < cut, f"{name}: mutants stay out of the independent checkers"
+        cpp = out / f"{name}.cpp"
+        cpp.write_text(src.replace(old, new))
+        so = out / f"{name}.so"
+        procs[name] = (so, subprocess.Popen(
+            ["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-w", "-shared", "-o", str(so),
+             str(cpp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+    libs = {}
+    for name, (so, p) in procs.items():
+        log = p.communicate()[0].decode(errors="replace")
+        assert p.returncode == 0, f"{name} failed to build:\n{log}"
+        libs[name] = str(so)
+    return libs
+
+
+@pytest.mark.parametrize("name,suite", [(m[0], m[1]) for m in MUTANTS])
+def test_pins_catch_planted_mistake(g2, orc, mutant_libs, monkeypatch, name, suite):
+    monkeypatch.setattr(orc, "_lib", orc.bind(mutant_libs[name]))
+    mod, names = (TM, MCE_PINS) if suite == "mce" else (TG, GM_PINS)
+    caught = []
+    for label, fn, kw in pin_cases(mod, names):
+        try:
+            fn(g2, orc, **kw)
+        except AssertionError:
+            caught.append(label)
+            break
+    assert caught, f"no pinning check fails with the planted mistake {name}"
+
+
+def test_less_conditions_never_read():
+    """The LESS branch of the condition check (gmatch.h:559-588) is not in
+    MUTANTS because no plan the reference builds reaches it: over every preset
+    and random fixture plan, LESS occurs only in the pre_cond rows of the root
+    and of positions that are not shared, which the search never reads.  A mistake there cannot
+    change a count, so no check could catch it."""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plans.json")) as f:
+        fx = json.load(f)
+    plans = list(fx["presets"].values()) + [r["plan"] for r in fx["random"]]
+    assert len(plans) == 72
+    less_rows = 0
+    for p in plans:
+        for key in ("cond", "after_cond"):
+            assert all(op != 0 for row in p[key] for op, _ in row), key
+        for u, row in enumerate(p["pre_cond"]):
+            if any(op == 0 for op, _ in row):
+                less_rows += 1
+                pos = p["rev"][u]
+                assert pos == 0 or not p["shared"][pos]
+    assert less_rows > 0
