@@ -77,3 +77,24 @@ def test_per_root_counts_sum(g2, orc):
     _, qadj = g2.query_preset(24)
     r = orc.gm(g, orc_plan(g2, orc, qadj), per_root=True)
     assert int(r["per_root_count"].sum()) == r["count"]
+
+
+def test_byte_model_hand_worked(g2, orc):
+    # Triangles (preset 0) on K_4.  Plan: position 1 extends from 0 with
+    # M1 > M0; position 2 from 0 and 1 (row of 0 first on the degree tie) with
+    # M2 > M0, M2 > M1; position 1 stores its list by EXPAND, 4(i+1) bytes per
+    # entry.  Per extension of a match of length i through parent row pi:
+    # 4i + 4d(pi) + c(1, d(b)) per candidate passing the conditions for each
+    # other back neighbour b, + the stored list.  All degrees are 3, so
+    # 4d = 12 and c(1,3) = 4 min(4, 3) = 12.
+    #   root 0: 4 | i=1: 4+12, list {1,2,3}: 24 | i=2 via 1: 8+12+2*12,
+    #           via 2: 8+12+12, via 3: 8+12                         = 140
+    #   root 1: 4 | i=1: 4+12, list {2,3}: 16 | via 2: 8+12+12, via 3: 20 = 88
+    #   root 2: 4 | i=1: 4+12, list {3}: 8 | via 3: 20                 = 48
+    #   root 3: 4 | i=1: 4+12, empty list                              = 20
+    # (Pricing the list as PREFIX, 4(i + |list|), would give 288.)
+    _, qadj = g2.query_preset(0)
+    p = orc_plan(g2, orc, qadj)
+    assert list(p.strategy)[:3] == [0, 0, 0] and list(p.bn[2])[:2] == [0, 1]
+    r = orc.gm(G.complete(g2, 4), p)
+    assert r["count"] == 4 and r["bytes"] == 140 + 88 + 48 + 20

@@ -116,3 +116,17 @@ def test_per_root_nodes_tie_rule(g2, orc):
     r = orc.mce(g, per_root=True)
     assert r["total"] == 3 and hist_dict(r["hist"]) == {3: 1, 4: 2}
     assert list(r["per_root_nodes"]) == [8, 1, 1, 1, 1, 1]
+
+
+def test_byte_model_hand_worked(g2, orc):
+    # Modelled bytes per node: 5s + sum_{u in P u X} c(P,u) + c(P,u*)
+    #   + sum_{q in Q} [c(P u X, q) + 5|child_q|],  c(a,d) = 4 min(a+d, a(1+bitlen d)).
+    # Path 0-1-2 (degrees 1, 2, 1):
+    #   root 0: [0r 1p] 10; candidate 1: c(1,2)=12; pivot 1: 12; Q={1}:
+    #           child [1r 0r]: c(1,2)+10 = 22; child node 10.          66
+    #   root 1: [1r 0x 2p] 15; candidates 0, 2: c(1,1)=8 each; pivot 2: 8;
+    #           Q={2}: child [2r 1r]: c(2,1)+10 = 12+10; child node 10.  71
+    #   root 2: [2r 1x] 10; candidate 1: c(0,2)=0; pivot 1: 0.          10
+    g = G.from_pairs(g2, [(0, 1), (1, 2)], 3)
+    r = orc.mce(g, per_root=True)
+    assert list(r["per_root_bytes"]) == [66, 71, 10] and r["bytes"] == 147

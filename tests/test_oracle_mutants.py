@@ -84,15 +84,39 @@ MUTANTS = [
     ("gm_last_level_counts_prefixes", "gm",
      "if (id + 1 == p->n) { c.count += cands.size(); return; }",
      "if (id + 1 == p->n) { c.count += !cands.empty(); return; }"),
+    # the byte model: c(A,u) = 4 min(|A| + d(u), |A| (1 + bitlen d(u)))
+    ("bytes_merge_cost_off_by_one", "mce",
+     "uint64_t merge = a + d;",
+     "uint64_t merge = a + d + 1;"),
+    # BK node: 5s for reading the node
+    ("mce_bytes_node_read", "mce",
+     "acc.bytes += 5 * s;",
+     "acc.bytes += 4 * s;"),
+    # BK node: c(P, u*) for Q = P minus N(u*)
+    ("mce_bytes_pivot_term_dropped", "mce",
+     "acc.bytes += cost_isect(pl.size(), phi - plo);",
+     "(void)phi;"),
+    # BK node: each child scans P u X against N(q)
+    ("mce_bytes_child_scan_over_p", "mce",
+     "acc.bytes += cost_isect(nPX, qhi - qlo) + 5 * child.size();",
+     "acc.bytes += cost_isect(pl.size(), qhi - qlo) + 5 * child.size();"),
+    # GM extension: membership tests per candidate passing the conditions
+    ("gm_bytes_membership_dropped", "gm",
+     "c.bytes += bcost;",
+     "(void)bcost;"),
+    # GM extension: an EXPAND list stores whole partial matches
+    ("gm_bytes_expand_priced_as_prefix", "gm",
+     "return 4 * (uint64_t)(id + 1) * nout;",
+     "return 4 * (id + nout);"),
 ]
 
 MCE_PINS = ["test_bruteforce", "test_complete_graph", "test_complete_bipartite", "test_cycle",
             "test_moon_moser", "test_turan", "test_disjoint_union_and_isolated",
             "test_relabel_invariance_and_root_partition", "test_per_root_nodes_tiny",
-            "test_per_root_nodes_tie_rule"]
+            "test_per_root_nodes_tie_rule", "test_byte_model_hand_worked"]
 GM_PINS = ["test_cliques_closed_form", "test_presets_vs_bruteforce",
            "test_random_patterns_vs_bruteforce", "test_kclique_independent_method",
-           "test_per_root_counts_sum"]
+           "test_per_root_counts_sum", "test_byte_model_hand_worked"]
 
 
 def pin_cases(mod, names):
