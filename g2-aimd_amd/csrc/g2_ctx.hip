@@ -145,6 +145,15 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     if (nv >= 0xffffffffull) return set_err(c, G2_EINVAL, "vertex ids must fit in u32");
     for (uint64_t i = 0; i < ns; i++)
         if (src[i] >= nv) return set_err(c, G2_EINVAL, "source id out of range");
+    // The kernels index rows and columns without bounds checks, so a malformed
+    // CSR is refused here, before anything on the device changes.
+    uint64_t md = 0;
+    for (uint64_t v = 0; v < nv; v++) {
+        if (rp[v + 1] < rp[v]) return set_err(c, G2_EINVAL, "row_ptrs must be non-decreasing");
+        md = std::max(md, rp[v + 1] - rp[v]);
+    }
+    for (uint64_t j = nv ? rp[0] : ne; j < ne; j++)
+        if (cols[j] >= nv) return set_err(c, G2_EINVAL, "column id out of range");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     int rc;
@@ -157,8 +166,6 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     c->nv = nv;
     c->ne = ne;
     c->ns = ns;
-    uint64_t md = 0;
-    for (uint64_t v = 0; v < nv; v++) md = std::max(md, rp[v + 1] - rp[v]);
     c->max_degree = md;
     // whole-graph view: every vertex is a source, in id order
     c->sources_all = (ns == nv);

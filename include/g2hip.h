@@ -99,7 +99,9 @@ void g2_destroy(g2_ctx* ctx);
 const char* g2_last_error(const g2_ctx* ctx);
 
 /* Copy a view (CSR rows indexed by global vertex id, sorted) and its source
- * vertices to the device.  The caller keeps ownership of the host arrays. */
+ * vertices to the device.  The caller keeps ownership of the host arrays.
+ * G2_EINVAL, with nothing on the device changed, when row_ptrs decreases,
+ * row_ptrs[nv] != ne, or a column or source id is >= nv. */
 int  g2_upload_view(g2_ctx* ctx, const uint64_t* row_ptrs, uint64_t nv,
                     const uint32_t* cols, uint64_t ne,
                     const uint32_t* sources, uint64_t ns);

@@ -190,6 +190,28 @@ def test_roots_per_wave(g2, orc, cfg1, rpw):
 
 
 
+def test_malformed_view_refused(g2, orc, gpu_exec):
+    """A column id >= nv or a decreasing row_ptrs is refused on the host,
+    before anything on the device changes: the view uploaded before stays
+    in place and still gives the oracle's counts."""
+    print("case: malformed view refused", flush=True)
+    g = G.gnp(g2, 50, 0.2, 4)
+    gpu_exec.transfer(g)
+    ref = orc.mce(g, per_root=True)
+    cols = g.cols.copy()
+    cols[-1] = g.nv
+    with pytest.raises(RuntimeError, match="column id out of range"):
+        gpu_exec.transfer(g2.Graph(g.row_ptrs, cols))
+    rp = g.row_ptrs.copy()
+    rp[10] = rp[11] + 1
+    assert rp[10] <= g.ne
+    with pytest.raises(RuntimeError, match="non-decreasing"):
+        gpu_exec.transfer(g2.Graph(rp, g.cols))
+    got = gpu_exec.run_mce(per_root=True)
+    assert got["total"] == ref["total"] and np.array_equal(got["hist"], ref["hist"])
+    assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])
+
+
 def test_schedule_checks_and_ranges(g2, orc, gpu_exec):
     """The device-built task list: a repeated or out-of-range source in the
     order is rejected; schedule ranges cover exactly their positions."""
