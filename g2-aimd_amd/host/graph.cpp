@@ -92,16 +92,30 @@ int g2h_graph_read_bin(const char* path, g2h_graph** out) {
     uint64_t hdr[4];
     if (std::fread(hdr, 8, 4, f) != 4) { std::fclose(f); set_error("short header"); return -2; }
     if (hdr[0] != 4 || hdr[1] != 8) { std::fclose(f); set_error("uintV/uintE size mismatch"); return -3; }
+    // The header's sizes must account for the file exactly before anything is
+    // allocated from them, so a damaged header cannot ask for a huge buffer.
+    const uint64_t nv = hdr[2], ne = hdr[3];
+    long long fsize = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+    const bool sizes_ok = nv < 0xffffffffull && ne <= (1ull << 60) && fsize >= 0 &&
+                          (uint64_t)fsize == 32 + 8 * (nv + 1) + 4 * ne;
+    if (!sizes_ok || std::fseek(f, 32, SEEK_SET) != 0) {
+        std::fclose(f);
+        set_error("malformed .bin: header sizes do not match the file");
+        return -4;
+    }
     auto* g = new g2h_graph;
-    g->nv = hdr[2];
-    g->ne = hdr[3];
-    g->rp.resize(g->nv + 1);
-    g->cols.resize(g->ne);
-    size_t r1 = std::fread(g->rp.data(), 8, g->nv + 1, f);
-    size_t r2 = std::fread(g->cols.data(), 4, g->ne, f);
-    int extra = std::fgetc(f);
+    g->nv = nv;
+    g->ne = ne;
+    g->rp.resize(nv + 1);
+    g->cols.resize(ne);
+    size_t r1 = std::fread(g->rp.data(), 8, nv + 1, f);
+    size_t r2 = std::fread(g->cols.data(), 4, ne, f);
     std::fclose(f);
-    if (r1 != g->nv + 1 || r2 != g->ne || extra != EOF || g->rp[g->nv] != g->ne) {
+    bool ok = r1 == nv + 1 && r2 == ne && g->rp[nv] == ne;
+    for (uint64_t v = 0; ok && v < nv; v++) ok = g->rp[v] <= g->rp[v + 1];
+    for (uint64_t j = 0; ok && j < ne; j++) ok = g->cols[j] < nv;
+    if (!ok) {
         delete g;
         set_error("malformed .bin body");
         return -4;

@@ -46,6 +46,41 @@ def test_bin_roundtrip_and_layout(g2, tmp_path):
         g2.Graph.read_bin(p)
 
 
+def test_bin_damaged_files_rejected(g2, tmp_path):
+    """A header whose sizes do not match the file is refused before anything
+    is allocated from it; a body with a decreasing row_ptrs or a column id
+    >= nv is refused after reading."""
+    g = G.gnp(g2, 40, 0.3, 2)
+    p = str(tmp_path / "g.bin")
+    g.write_bin(p)
+    raw = bytearray(open(p, "rb").read())
+
+    def rejected(data, match):
+        q = str(tmp_path / "bad.bin")
+        with open(q, "wb") as f:
+            f.write(bytes(data))
+        with pytest.raises(RuntimeError, match=match):
+            g2.Graph.read_bin(q)
+
+    huge = bytearray(raw)
+    huge[24:32] = np.array([1 << 59], np.uint64).tobytes()    # ne far beyond the file
+    rejected(huge, "header sizes")
+    rejected(raw[:-4], "header sizes")                         # truncated body
+    nvbig = bytearray(raw)
+    nvbig[16:24] = np.array([1 << 40], np.uint64).tobytes()
+    rejected(nvbig, "header sizes")
+    col = bytearray(raw)
+    off = 32 + 8 * (g.nv + 1)
+    col[off:off + 4] = np.array([g.nv], np.uint32).tobytes()  # column id == nv
+    rejected(col, "malformed .bin body")
+    rp = bytearray(raw)
+    v = 5
+    bad = int(g.row_ptrs[v + 1]) + 1
+    rp[32 + 8 * v:32 + 8 * (v + 1)] = np.array([bad], np.uint64).tobytes()
+    rejected(rp, "malformed .bin body")
+    assert same(g2.Graph.read_bin(p), g)
+
+
 @pytest.mark.parametrize("k", [0, 1, 2])
 def test_preprocess_matches_reference(g2, k):
     g = read_snap(g2, os.path.join(GOLD, "toy_%d.txt" % k)).preprocess()
