@@ -292,7 +292,7 @@ int g2h_write_vbmap(const char* path, uint32_t nbins, const uint32_t* bins, uint
 int g2h_read_vbmap(const char* path, uint32_t* nbins, uint32_t* bins, uint64_t nv) {
     std::ifstream in(path);
     if (!in) { set_error(std::string("cannot open ") + path); return -1; }
-    in >> *nbins;
+    if (!(in >> *nbins)) { set_error("vbmap has no bin count"); return -2; }
     for (uint64_t v = 0; v < nv; v++) {
         if (!(in >> bins[v])) { set_error("vbmap shorter than vertex count"); return -2; }
         if (bins[v] >= *nbins) { set_error("vbmap bin id out of range"); return -3; }

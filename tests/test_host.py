@@ -100,6 +100,13 @@ def test_kcore_matches_reference(g2, k, tmp_path):
     assert open(p).read().split() == open(os.path.join(GOLD, "toy_%d.kcore.vbmap" % k)).read().split()
     nb, back = g2.read_vbmap(p, g.nv)
     assert nb == 4 and np.array_equal(back, bins)
+    bad = str(tmp_path / "bad.vbmap")
+    for text, match in (("", "no bin count"), ("4 0 1", "shorter"),
+                        ("2 " + "0 " * (g.nv - 1) + "2", "out of range")):
+        with open(bad, "w") as f:
+            f.write(text)
+        with pytest.raises(RuntimeError, match=match):
+            g2.read_vbmap(bad, g.nv)
 
 
 def test_kcore_order_is_a_peeling_order(g2):
