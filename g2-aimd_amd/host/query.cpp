@@ -26,6 +26,18 @@ namespace {
 
 using Rows = std::vector<std::vector<uint32_t>>;
 
+// A pattern is 1..G2_MAXQ vertices whose adjacency masks are symmetric, with
+// no self loops and no bits at or above n.
+bool valid_pattern(uint32_t n, const uint32_t* qadj) {
+    if (n < 1 || n > G2_MAXQ) return false;
+    for (uint32_t u = 0; u < n; u++) {
+        if ((qadj[u] >> u & 1) || (qadj[u] >> n) != 0) return false;
+        for (uint32_t v = 0; v < n; v++)
+            if ((qadj[u] >> v & 1) != (qadj[v] >> u & 1)) return false;
+    }
+    return true;
+}
+
 Rows rows_of(uint32_t n, const uint32_t* qadj) {
     Rows r(n);
     for (uint32_t u = 0; u < n; u++)
@@ -220,6 +232,7 @@ int g2h_query_preset(int preset, uint32_t* n, uint32_t qadj[G2_MAXQ]) {
 
 int g2h_query_remap(uint32_t n, const uint32_t qadj_in[G2_MAXQ], uint32_t qadj_out[G2_MAXQ]) {
     if (n < 1 || n > G2_MAXQ) { g2h::set_error("pattern size out of range"); return -1; }
+    if (!valid_pattern(n, qadj_in)) { g2h::set_error("pattern adjacency must be symmetric, without self loops"); return -1; }
     Rows m = remap(rows_of(n, qadj_in));
     for (uint32_t u = 0; u < G2_MAXQ; u++) qadj_out[u] = 0;
     for (uint32_t u = 0; u < n; u++)
@@ -230,6 +243,7 @@ int g2h_query_remap(uint32_t n, const uint32_t qadj_in[G2_MAXQ], uint32_t qadj_o
 int g2h_query_conditions(uint32_t n, const uint32_t qadj[G2_MAXQ], uint32_t* npairs,
                          uint32_t pairs[2 * G2_MAXQ * G2_MAXQ]) {
     if (n < 1 || n > G2_MAXQ) { g2h::set_error("pattern size out of range"); return -1; }
+    if (!valid_pattern(n, qadj)) { g2h::set_error("pattern adjacency must be symmetric, without self loops"); return -1; }
     auto c = conditions(rows_of(n, qadj));
     *npairs = c.size();
     for (size_t k = 0; k < c.size(); k++) { pairs[2 * k] = c[k].first; pairs[2 * k + 1] = c[k].second; }
@@ -237,6 +251,7 @@ int g2h_query_conditions(uint32_t n, const uint32_t qadj[G2_MAXQ], uint32_t* npa
 }
 
 uint64_t g2h_query_automorphisms(uint32_t n, const uint32_t qadj[G2_MAXQ]) {
+    if (!valid_pattern(n, qadj)) return 0;
     return automorphisms(rows_of(n, qadj)).size();
 }
 
@@ -244,7 +259,7 @@ uint64_t g2h_query_automorphisms(uint32_t n, const uint32_t qadj[G2_MAXQ]) {
 // string over all relabellings (n <= 8, so at most 8! = 40320 of them), with
 // n in the top byte.  Equal codes <=> isomorphic patterns.
 uint64_t g2h_query_canonical(uint32_t n, const uint32_t qadj[G2_MAXQ]) {
-    if (n < 1 || n > G2_MAXQ) return 0;
+    if (!valid_pattern(n, qadj)) return 0;
     std::vector<uint32_t> perm(n);
     std::iota(perm.begin(), perm.end(), 0u);
     uint64_t best = ~0ull;
@@ -261,6 +276,7 @@ uint64_t g2h_query_canonical(uint32_t n, const uint32_t qadj[G2_MAXQ]) {
 
 int g2h_plan_build(uint32_t n, const uint32_t qadj[G2_MAXQ], g2_plan* P) {
     if (n < 2 || n > G2_MAXQ) { g2h::set_error("pattern size out of range"); return -1; }
+    if (!valid_pattern(n, qadj)) { g2h::set_error("pattern adjacency must be symmetric, without self loops"); return -1; }
     const Rows r = rows_of(n, qadj);
     for (uint32_t u = 0; u < n; u++)
         if (r[u].empty()) { g2h::set_error("pattern is not connected"); return -2; }

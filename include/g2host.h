@@ -92,9 +92,12 @@ int  g2h_query_remap(uint32_t n, const uint32_t qadj_in[G2_MAXQ], uint32_t qadj_
  * group, enumerated directly instead of through bliss. pairs[2*k], pairs[2*k+1] */
 int  g2h_query_conditions(uint32_t n, const uint32_t qadj[G2_MAXQ], uint32_t* npairs,
                           uint32_t pairs[2 * G2_MAXQ * G2_MAXQ]);
+/* |Aut| of the pattern; 0 when qadj is not a valid pattern (n in 1..G2_MAXQ,
+ * symmetric masks, no self loops, no bits at or above n) */
 uint64_t g2h_query_automorphisms(uint32_t n, const uint32_t qadj[G2_MAXQ]);
-/* isomorphism-invariant code of a pattern (equal codes <=> isomorphic);
- * no counterpart in the reference: lets a batch count each class once */
+/* isomorphism-invariant code of a pattern (equal codes <=> isomorphic), 0
+ * for an invalid pattern; no counterpart in the reference: lets a batch count
+ * each class once */
 uint64_t g2h_query_canonical(uint32_t n, const uint32_t qadj[G2_MAXQ]);
 /* full plan of system/plan.h:51-372 for a remapped pattern */
 int  g2h_plan_build(uint32_t n, const uint32_t qadj[G2_MAXQ], g2_plan* plan);

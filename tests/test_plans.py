@@ -46,3 +46,20 @@ def test_random_pattern_generator(g2):
 def test_disconnected_pattern_rejected(g2):
     with pytest.raises(RuntimeError):
         g2.plan_build([0b10, 0b01, 0b1000, 0b0100])
+
+
+def test_malformed_patterns_rejected(g2):
+    """Adjacency masks must be symmetric, without self loops or bits at or
+    above n; the plan and condition builders refuse anything else, and the
+    automorphism count and canonical code are 0 for it."""
+    for bad in ([0b10, 0b00, 0b00],          # 0-1 listed on one side only
+                [0b11, 0b01],                # self loop at 0
+                [0b1010, 0b0001, 0b0000]):   # bit 3 with n = 3
+        with pytest.raises(RuntimeError, match="symmetric"):
+            g2.plan_build(bad)
+        with pytest.raises(RuntimeError, match="symmetric"):
+            g2.query_conditions(bad)
+        with pytest.raises(RuntimeError, match="symmetric"):
+            g2.query_remap(bad)
+        assert g2.query_automorphisms(bad) == 0 and g2.query_canonical(bad) == 0
+    assert g2.query_automorphisms([0b110, 0b101, 0b011]) == 6
