@@ -472,12 +472,34 @@ static uint64_t fit_arenas(g2_ctx* c, uint64_t nblocks, uint64_t per_block) {
 int gm_compile(g2_ctx* c, const g2_plan* p) {
     if (p->n < 2 || p->n > G2_MAXQ) return set_err(c, G2_EINVAL, "plan size must be 2..8");
     const uint32_t n = p->n;
+    // Every table below is indexed through seq/rev and sized G2_MAXQ: a plan
+    // that breaks these bounds is refused before any entry is read through it.
+    for (uint32_t i = 0; i < n; i++)
+        if (p->seq[i] >= n || p->rev[p->seq[i]] != i)
+            return set_err(c, G2_EINVAL, "seq and rev are not inverse permutations");
+    auto vertices_ok = [n](uint32_t cnt, const uint32_t* v) {
+        if (cnt > G2_MAXQ) return false;
+        for (uint32_t k = 0; k < cnt; k++)
+            if (v[k] >= n) return false;
+        return true;
+    };
+    auto conds_ok = [n](uint32_t cnt, const uint32_t (*cd)[2]) {
+        if (cnt > G2_MAXQ) return false;
+        for (uint32_t k = 0; k < cnt; k++)
+            if (cd[k][1] >= n) return false;
+        return true;
+    };
     GmProgram g{};
     g.n = n;
     for (uint32_t id = 1; id < n; id++) {
         GmStep& s = g.step[id];
         const uint32_t u = p->seq[id];
         if (p->bn_count[id] == 0 || p->bn_count[id] > G2_MAXQ) return set_err(c, G2_EINVAL, "position without back neighbour");
+        if (!conds_ok(p->cond_count[u], p->cond[u])) return set_err(c, G2_EINVAL, "bad condition");
+        if (p->shared[id] &&
+            (!vertices_ok(p->pre_bn_count[id], p->pre_bn[id]) || !vertices_ok(p->after_bn_count[id], p->after_bn[id]) ||
+             !conds_ok(p->pre_cond_count[u], p->pre_cond[u]) || !conds_ok(p->after_cond_count[u], p->after_cond[u])))
+            return set_err(c, G2_EINVAL, "shared position tables out of range");
         s.nbn = p->bn_count[id];
         for (uint32_t k = 0; k < s.nbn; k++) {
             const uint32_t b = p->bn[id][k];
