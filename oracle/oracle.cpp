@@ -46,7 +46,10 @@
 // cycles, Turan graphs, disjoint unions), brute-force injective
 // homomorphisms / |Aut| (orc_bruteforce_pattern), the reference's own plan
 // tables compiled from its sources (oracle/_ref), and README.md:65-66 style
-// clique counts by an independent node-iterator (orc_kclique_count).
+// clique counts by an independent node-iterator (orc_kclique_count); per-root
+// node counts and modelled bytes worked by hand on small graphs.  The tests
+// also plant mistakes in copies of this file and require the pins to catch
+// each one.
 // ============================================================================
 #include <algorithm>
 #include <atomic>
