@@ -512,6 +512,8 @@ def main():
                 "per_s": r["total"] / (np.mean(ts) / 1e3),
                 "alg_bytes": acc_m["stats"]["alg_bytes"],
                 "achieved_GBs": acc_m["stats"]["alg_bytes"] / (kms / 1e3) / 1e9,
+                "note": "kernel_ms is the last of the 5 timed runs; the 9 MB CSR stays in "
+                        "L2 and MALL, so achieved_GBs (modelled bytes) says nothing about HBM",
             }
         out["extra"] = extra
     else:
