@@ -207,6 +207,7 @@ int run_worker(int argc, char** argv, App app) {
     const int gpus = g2_device_count();
     say("GPU count: " + std::to_string(gpus));
     if (gpus < 1) die("no GPU visible");
+    if (pn < 1 || cn < 1) die("producer and consumer counts must be >= 1");
     if (qs < cn) die("queue size must be >= consumer count");
     {
         std::ostringstream o;
