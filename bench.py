@@ -432,7 +432,7 @@ def main():
         log(f"copied {n * 4} bytes 3 times")
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); "
               "the hot path would shard by root ranges, with one u64 count (and the clique "
               "histogram) summed over GPUs", file=sys.stderr)
         sys.exit(2)
