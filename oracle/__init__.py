@@ -1,7 +1,7 @@
 """oracle -- ctypes binding of the CPU restatement (oracle/oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+bench.py's cpu_baseline leg, to check results.  The product never imports it.
 """
 from __future__ import annotations
 

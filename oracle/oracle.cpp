@@ -1,7 +1,7 @@
 // ============================================================================
 // oracle/oracle.cpp -- CPU restatement of G2-AIMD's subgraph-search hot path.
 //
-// TEST INFRASTRUCTURE ONLY.  This library is the parity checker: it may be
+// TEST INFRASTRUCTURE ONLY.  This library is the parity oracle: it may be
 // loaded by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
 // leg, and by nothing else.  The product (libg2hip.so, the bk/run CLIs) never
 // links, loads or calls it.
