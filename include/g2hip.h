@@ -110,7 +110,10 @@ int  g2_upload_view(g2_ctx* ctx, const uint64_t* row_ptrs, uint64_t nv,
  * Bron-Kerbosch, app_BK/BK.h).  total = number of maximal cliques;
  * hist[k] = cliques with k vertices (k >= hist_len-1 land in the last bin);
  * per_root_nodes (nullable, ns entries, source order) = search-tree nodes of
- * each root's subtree. */
+ * each root's subtree.  G2_EBUDGET when a root has more than 4096 neighbours
+ * above it (P sets are at most 64 words per lane) or its arena does not fit
+ * the budget; degeneracy-ordered inputs keep the first bound at the
+ * degeneracy. */
 int  g2_mce_run(g2_ctx* ctx, const g2_sched* sched, uint32_t flags,
                 uint64_t* total, uint64_t* hist, uint32_t hist_len,
                 uint64_t* per_root_nodes, g2_stats* stats);
