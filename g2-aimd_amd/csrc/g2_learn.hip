@@ -86,7 +86,16 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
 int features_run(g2_ctx* c, int64_t* out) {
     int rc;
     if ((rc = ensure(c, c->counters, 8, "counters"))) return rc;
-    DevBuf dout;
+    // the output buffer lives for this call only, on every return path
+    struct CallBuf {
+        g2_ctx* c;
+        DevBuf b;
+        ~CallBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+        }
+    } out_buf{c, {}};
+    DevBuf& dout = out_buf.b;
     if ((rc = ensure(c, dout, std::max<uint64_t>(c->ns, 1) * G2_NFEAT_INT * 8, "features"))) return rc;
     hipError_t e;
     if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
@@ -98,8 +107,6 @@ int features_run(g2_ctx* c, int64_t* out) {
     if (c->ns && (e = hipMemcpyAsync(out, dout.ptr, c->ns * G2_NFEAT_INT * 8, hipMemcpyDeviceToHost, c->stream)))
         return hip_err(c, e, "D2H features");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "feature sync");
-    (void)hipFree(dout.ptr);
-    c->used -= dout.bytes;
     return G2_OK;
 }
 
@@ -408,8 +415,16 @@ using namespace g2;
 
 static int mlp_ensure(g2_mlp* M, uint64_t n) {
     if (n <= M->cap) return G2_OK;
-    for (void* p : {(void*)M->x, (void*)M->y, (void*)M->out, (void*)M->idx})
-        if (p) (void)hipFree(p);
+    // drop the old buffers first; after a failed allocation every pointer is
+    // either live or null, so a later call or the destroy frees each once
+    float** fp[] = {&M->x, &M->y, &M->out};
+    for (float** p : fp) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (M->idx) (void)hipFree(M->idx);
+    M->idx = nullptr;
+    M->cap = 0;
     if (hipMalloc(&M->x, n * kIn * 4) || hipMalloc(&M->y, n * 4) || hipMalloc(&M->out, n * 4) ||
         hipMalloc(&M->idx, n * 4))
         return G2_EHIP;
