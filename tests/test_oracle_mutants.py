@@ -5,14 +5,17 @@ Each case below plants one plausible mistake in a copy of oracle/oracle.cpp
 copy on the CPU, and runs the pinning tests of test_oracle_mce.py or
 test_oracle_gm.py against it.  At least one of them must fail.  The mistakes
 are planted only in the restatements (orc_mce, orc_gm and their helpers), never
-in the independent checkers they are compared with.
+in the independent checkers they are compared with.  The cost model's CPU
+reference (oracle/mlp_ref.py) gets the same treatment against test_mlp_ref.py.
 """
+import importlib.util
 import json
 import os
 import subprocess
 
 import pytest
 
+import test_mlp_ref as TL
 import test_oracle_gm as TG
 import test_oracle_mce as TM
 
@@ -193,3 +196,40 @@ def test_less_conditions_never_read():
                 pos = p["rev"][u]
                 assert pos == 0 or not p["shared"][pos]
     assert less_rows > 0
+
+
+MLP_SRC = os.path.join(os.path.dirname(ORACLE_SRC), "mlp_ref.py")
+
+# planted mistakes in the cost model's CPU reference (oracle/mlp_ref.py)
+MLP_MUTANTS = [
+    ("mlp_forward_drops_b2", "z2 = h1 @ r(W2) + b2", "z2 = h1 @ r(W2)"),
+    ("mlp_backward_w2_not_transposed", "dh1 = (dh2 @ r(W2).T) * (z1 > 0)", "dh1 = (dh2 @ r(W2)) * (z1 > 0)"),
+    ("mlp_backward_drops_layer2_relu_mask", "dh2 = np.outer(dy, r(W3)) * (z2 > 0)", "dh2 = np.outer(dy, r(W3))"),
+    ("mlp_loss_gradient_not_averaged", "dy = 2 * e / n", "dy = 2 * e"),
+    ("bf16_truncates", "u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16", "u = (u >> 16) << 16"),
+    ("bf16_ties_away_from_zero", "u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16",
+     "u = ((u + 0x8000) >> 16) << 16"),
+]
+MLP_PINS = ["test_fd_gradient", "test_bf16_rounding", "test_forward_and_gradient_match_torch",
+            "test_bf16_rounding_matches_torch"]
+
+
+@pytest.mark.parametrize("name,old,new", MLP_MUTANTS)
+def test_mlp_pins_catch_planted_mistake(tmp_path, monkeypatch, name, old, new):
+    with open(MLP_SRC) as f:
+        src = f.read()
+    assert src.count(old) == 1, f"{name}: the original text must occur exactly once"
+    path = tmp_path / f"{name}.py"
+    path.write_text(src.replace(old, new))
+    spec = importlib.util.spec_from_file_location(f"mlp_ref_{name}", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    monkeypatch.setattr(TL, "mlp_ref", mod)
+    caught = []
+    for pin in MLP_PINS:
+        try:
+            getattr(TL, pin)()
+        except AssertionError:
+            caught.append(pin)
+            break
+    assert caught, f"no pinning check fails with the planted mistake {name}"
