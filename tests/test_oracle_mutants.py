@@ -107,6 +107,13 @@ MUTANTS = [
     ("gm_bytes_membership_dropped", "gm",
      "c.bytes += bcost;",
      "(void)bcost;"),
+    # per-root outputs are each root's own subtree, not a running total
+    ("mce_per_root_nodes_cumulative", "mce",
+     "if (per_root_nodes) per_root_nodes[i] = acc.nodes - n0;",
+     "if (per_root_nodes) per_root_nodes[i] = acc.nodes;"),
+    ("gm_per_root_counts_cumulative", "gm",
+     "if (per_root_count) per_root_count[i] = c.count - c0;",
+     "if (per_root_count) per_root_count[i] = c.count;"),
     # GM extension: an EXPAND list stores whole partial matches
     ("gm_bytes_expand_priced_as_prefix", "gm",
      "return 4 * (uint64_t)(id + 1) * nout;",
