@@ -20,7 +20,7 @@ using namespace g2;
 extern "C" int g2_gm_autotune(g2_ctx* c, const g2_plan* plans, uint32_t nq, uint32_t sample_queries,
                               uint64_t seed, g2_batch* best, double* best_ms) {
     if (!c || !best) return G2_EINVAL;
-    if (nq && !plans) return set_err(c, G2_EINVAL, "null plans");
+    if (nq == 0 || !plans) return set_err(c, G2_EINVAL, "no queries to tune on");
     const uint64_t budget = best->node_budget;
     std::vector<uint32_t> pick(nq);
     std::iota(pick.begin(), pick.end(), 0u);
