@@ -128,7 +128,7 @@ void g2_destroy(g2_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->rp, &c->cols, &c->src, &c->gm_task_off, &c->counters, &c->scratch,
                       &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans,
-                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate, &c->taskbuild, &c->orient})
+                      &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate, &c->taskbuild, &c->orient, &c->roots})
         release(c, *b);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -154,15 +154,60 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     }
     for (uint64_t j = nv ? rp[0] : ne; j < ne; j++)
         if (cols[j] >= nv) return set_err(c, G2_EINVAL, "column id out of range");
+    // Level-1 windows of the matching tasks: source i covers windows
+    // [off[i], off[i+1]) of gm_chunk row entries each (one window when 0).
+    // Every neighbour of a source having its row in the view (a view of hop
+    // >= 2, or the whole graph) is what the local-bitmap matching path needs.
+    std::vector<uint64_t> task_off(ns + 1, 0), src_deg(ns);
+    const uint32_t chunk = c->tune.gm_chunk;
+    bool closed = true;
+    for (uint64_t i = 0; i < ns; i++) {
+        const uint64_t d = rp[src[i] + 1] - rp[src[i]];
+        src_deg[i] = d;
+        uint64_t w = 1;
+        if (chunk && d > chunk) w = (d + chunk - 1) / chunk;
+        task_off[i + 1] = task_off[i] + w;
+        for (uint64_t j = rp[src[i]]; closed && j < rp[src[i] + 1]; j++) closed = rp[cols[j] + 1] > rp[cols[j]];
+    }
+    // The budget is checked for every view buffer before any is replaced, so
+    // a refused upload leaves the previous view intact.
+    struct Need { DevBuf* b; uint64_t bytes; const char* what; };
+    const Need needs[] = {{&c->rp, (nv + 1) * 8, "row_ptrs"},
+                          {&c->cols, std::max<uint64_t>(ne, 1) * 4, "cols"},
+                          {&c->src, std::max<uint64_t>(ns, 1) * 4, "sources"},
+                          {&c->deg, (nv + 1) * 4, "degrees"},
+                          {&c->gm_task_off, (ns + 1) * 8, "task offsets"}};
+    uint64_t after = c->used;
+    for (const Need& n : needs) {
+        const uint64_t want = (n.bytes + 255) & ~255ull;
+        if (want > n.b->bytes) after += want - n.b->bytes;
+    }
+    if (c->budget && after > c->budget)
+        return set_err(c, G2_EBUDGET, "view needs " + std::to_string(after - c->used) + " B more device memory, budget left " +
+                                          std::to_string(c->budget - std::min(c->budget, c->used)));
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    // From here on a failure leaves no view: runs return G2_ENOVIEW until an
+    // upload succeeds.
+    c->view_ok = false;
+    c->nv = c->ne = c->ns = 0;
+    auto fail = [&](int rc) {
+        release(c, c->rp);
+        return rc;
+    };
     int rc;
-    if ((rc = ensure(c, c->rp, (nv + 1) * 8, "row_ptrs"))) return rc;
-    if ((rc = ensure(c, c->cols, std::max<uint64_t>(ne, 1) * 4, "cols"))) return rc;
-    if ((rc = ensure(c, c->src, std::max<uint64_t>(ns, 1) * 4, "sources"))) return rc;
-    if ((e = hipMemcpyAsync(c->rp.ptr, rp, (nv + 1) * 8, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D row_ptrs");
-    if (ne && (e = hipMemcpyAsync(c->cols.ptr, cols, ne * 4, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D cols");
-    if (ns && (e = hipMemcpyAsync(c->src.ptr, src, ns * 4, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D sources");
+    for (const Need& n : needs)
+        if ((rc = ensure(c, *n.b, n.bytes, n.what))) return fail(rc);
+    std::vector<uint32_t> dg(nv + 1, 0);
+    for (uint64_t v = 0; v < nv; v++) dg[v] = (uint32_t)std::min<uint64_t>(rp[v + 1] - rp[v], 0xffffffffull);
+    if ((e = hipMemcpyAsync(c->rp.ptr, rp, (nv + 1) * 8, hipMemcpyHostToDevice, c->stream))) return fail(hip_err(c, e, "H2D row_ptrs"));
+    if (ne && (e = hipMemcpyAsync(c->cols.ptr, cols, ne * 4, hipMemcpyHostToDevice, c->stream))) return fail(hip_err(c, e, "H2D cols"));
+    if (ns && (e = hipMemcpyAsync(c->src.ptr, src, ns * 4, hipMemcpyHostToDevice, c->stream))) return fail(hip_err(c, e, "H2D sources"));
+    if ((e = hipMemcpyAsync(c->deg.ptr, dg.data(), (nv + 1) * 4, hipMemcpyHostToDevice, c->stream)))
+        return fail(hip_err(c, e, "H2D degrees"));
+    if ((e = hipMemcpyAsync(c->gm_task_off.ptr, task_off.data(), (ns + 1) * 8, hipMemcpyHostToDevice, c->stream)))
+        return fail(hip_err(c, e, "H2D task offsets"));
+    if ((e = hipStreamSynchronize(c->stream))) return fail(hip_err(c, e, "upload sync"));
     c->nv = nv;
     c->ne = ne;
     c->ns = ns;
@@ -170,36 +215,19 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     // whole-graph view: every vertex is a source, in id order
     c->sources_all = (ns == nv);
     for (uint64_t i = 0; c->sources_all && i < ns; i++) c->sources_all = (src[i] == i);
-    {
-        std::vector<uint32_t> dg(nv + 1, 0);
-        for (uint64_t v = 0; v < nv; v++) dg[v] = (uint32_t)std::min<uint64_t>(rp[v + 1] - rp[v], 0xffffffffull);
-        if ((rc = ensure(c, c->deg, (nv + 1) * 4, "degrees"))) return rc;
-        if ((e = hipMemcpy(c->deg.ptr, dg.data(), (nv + 1) * 4, hipMemcpyHostToDevice))) return hip_err(c, e, "H2D degrees");
-    }
-    // level-1 windows of the matching tasks: source i covers windows
-    // [off[i], off[i+1]) of gm_chunk row entries each (one window when 0)
-    c->gm_task_chunk = c->tune.gm_chunk;
-    c->h_gm_task_off.assign(ns + 1, 0);
-    c->h_src_deg.resize(ns);
+    c->sources_closed = closed;
+    c->gm_task_chunk = chunk;
+    c->h_gm_task_off.swap(task_off);
+    c->h_src_deg.swap(src_deg);
     c->batch_off_valid = false;
-    for (uint64_t i = 0; i < ns; i++) {
-        const uint64_t d = rp[src[i] + 1] - rp[src[i]];
-        c->h_src_deg[i] = d;
-        uint64_t w = 1;
-        if (c->gm_task_chunk && d > c->gm_task_chunk) w = (d + c->gm_task_chunk - 1) / c->gm_task_chunk;
-        c->h_gm_task_off[i + 1] = c->h_gm_task_off[i] + w;
-    }
-    if ((rc = ensure(c, c->gm_task_off, (ns + 1) * 8, "task offsets"))) return rc;
-    if ((e = hipMemcpyAsync(c->gm_task_off.ptr, c->h_gm_task_off.data(), (ns + 1) * 8, hipMemcpyHostToDevice, c->stream)))
-        return hip_err(c, e, "H2D task offsets");
-    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "upload sync");
+    c->view_ok = true;
     return G2_OK;
 }
 
 int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
                uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
     if (!c) return G2_EINVAL;
-    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
     if (hist && hist_len < 2) return set_err(c, G2_EINVAL, "hist_len must be >= 2");
     if (sched && sched->begin && sched->begin >= sched->end)
         return set_err(c, G2_EINVAL, "empty schedule range");
@@ -209,16 +237,18 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
     if (sched && sched->order && sched->n != c->ns) return set_err(c, G2_EINVAL, "schedule length != source count");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    if (st) *st = g2_stats{};
     return mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);
 }
 
 int g2_gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
                     uint64_t* nodes, uint32_t* over_budget, g2_stats* st) {
     if (!c) return G2_EINVAL;
-    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
     if (nq && !plans) return set_err(c, G2_EINVAL, "null plans");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    if (st) *st = g2_stats{};
     return gm_run_batch(c, plans, nq, cfg, counts, nodes, over_budget, st);
 }
 
@@ -236,10 +266,11 @@ int g2_gm_set_plan(g2_ctx* c, const g2_plan* p) {
 
 int g2_gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_count, g2_stats* st) {
     if (!c) return G2_EINVAL;
-    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
     if (!c->has_plan) return set_err(c, G2_ENOPLAN, "no plan set");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    if (st) *st = g2_stats{};
     return gm_run(c, flags, count, per_root_count, st);
 }
 

@@ -27,28 +27,13 @@
 
 namespace g2 {
 
-enum { GCTR_NEXT = 0, GCTR_COUNT, GCTR_BYTES, GCTR_NODES, GCTR_N };
 
-struct GmArgs {
-    const uint64_t* __restrict__ rp;
-    const uint32_t* __restrict__ cols;
-    const uint32_t* __restrict__ src;
-    const uint64_t* __restrict__ task_off;  // ns+1 window offsets per source
-    uint64_t ns;
-    uint64_t ntasks;
-    uint32_t chunk;                         // level-1 window length (0 = whole row)
-    uint32_t* scratch;
-    uint64_t cap;                           // entries per list
-    unsigned long long* ctr;
-    unsigned long long* per_root;           // nullable
-};
 
 struct Range {
     uint32_t lo;   // smallest admissible value
     uint32_t hi;   // one past the largest admissible value (0xffffffff = none)
 };
 
-constexpr int kMaxBlockWaves = 4;
 
 struct WaveState {
     uint32_t M[G2_MAXQ];
@@ -363,6 +348,10 @@ __device__ bool gm_root(const GmArgs& a, const GmProgram& P, WaveState& S, uint3
 
 // task t of a query -> (source, level-1 window)
 __device__ inline uint64_t task_source(const GmArgs& a, uint64_t t, uint64_t* win) {
+    if (!a.task_off) {  // whole rows: task t is source t
+        *win = 0;
+        return t;
+    }
     uint64_t lo = 0, hi = a.ns;
     while (hi - lo > 1) {
         const uint64_t mid = (lo + hi) >> 1;
@@ -375,7 +364,7 @@ __device__ inline uint64_t task_source(const GmArgs& a, uint64_t t, uint64_t* wi
 
 template <bool ACCOUNT>
 __global__ __launch_bounds__(256) void gm_kernel(GmArgs a, GmProgram prog) {
-    __shared__ WaveState states[kMaxBlockWaves];
+    __shared__ WaveState states[kGmMaxBlockWaves];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x / kWave;
     WaveState& S = states[wib];
@@ -421,7 +410,7 @@ struct GmBatchArgs {
 };
 
 __global__ __launch_bounds__(256) void gm_batch_kernel(GmArgs a, GmBatchArgs b) {
-    __shared__ WaveState states[kMaxBlockWaves];
+    __shared__ WaveState states[kGmMaxBlockWaves];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x / kWave;
     WaveState& S = states[wib];
@@ -469,14 +458,16 @@ static uint64_t fit_arenas(g2_ctx* c, uint64_t nblocks, uint64_t per_block) {
     return nblocks;
 }
 
-int gm_compile(g2_ctx* c, const g2_plan* p) {
-    if (p->n < 2 || p->n > G2_MAXQ) return set_err(c, G2_EINVAL, "plan size must be 2..8");
+// Plan tables (system/plan.h) re-indexed by position.  Returns G2_OK, or
+// G2_EINVAL with *why naming the first broken table.
+int gm_program(const g2_plan* p, GmProgram* out, const char** why) {
+    if (p->n < 2 || p->n > G2_MAXQ) return (*why = "plan size must be 2..8", G2_EINVAL);
     const uint32_t n = p->n;
     // Every table below is indexed through seq/rev and sized G2_MAXQ: a plan
     // that breaks these bounds is refused before any entry is read through it.
     for (uint32_t i = 0; i < n; i++)
         if (p->seq[i] >= n || p->rev[p->seq[i]] != i)
-            return set_err(c, G2_EINVAL, "seq and rev are not inverse permutations");
+            return (*why = "seq and rev are not inverse permutations", G2_EINVAL);
     auto vertices_ok = [n](uint32_t cnt, const uint32_t* v) {
         if (cnt > G2_MAXQ) return false;
         for (uint32_t k = 0; k < cnt; k++)
@@ -494,53 +485,53 @@ int gm_compile(g2_ctx* c, const g2_plan* p) {
     for (uint32_t id = 1; id < n; id++) {
         GmStep& s = g.step[id];
         const uint32_t u = p->seq[id];
-        if (p->bn_count[id] == 0 || p->bn_count[id] > G2_MAXQ) return set_err(c, G2_EINVAL, "position without back neighbour");
-        if (!conds_ok(p->cond_count[u], p->cond[u])) return set_err(c, G2_EINVAL, "bad condition");
+        if (p->bn_count[id] == 0 || p->bn_count[id] > G2_MAXQ) return (*why = "position without back neighbour", G2_EINVAL);
+        if (!conds_ok(p->cond_count[u], p->cond[u])) return (*why = "bad condition", G2_EINVAL);
         if (p->shared[id] &&
             (!vertices_ok(p->pre_bn_count[id], p->pre_bn[id]) || !vertices_ok(p->after_bn_count[id], p->after_bn[id]) ||
              !conds_ok(p->pre_cond_count[u], p->pre_cond[u]) || !conds_ok(p->after_cond_count[u], p->after_cond[u])))
-            return set_err(c, G2_EINVAL, "shared position tables out of range");
+            return (*why = "shared position tables out of range", G2_EINVAL);
         s.nbn = p->bn_count[id];
         for (uint32_t k = 0; k < s.nbn; k++) {
             const uint32_t b = p->bn[id][k];
-            if (b >= n || p->rev[b] >= id) return set_err(c, G2_EINVAL, "back neighbour not matched earlier");
+            if (b >= n || p->rev[b] >= id) return (*why = "back neighbour not matched earlier", G2_EINVAL);
             s.bn[k] = p->rev[b];
         }
         s.ncond = p->cond_count[u];
         for (uint32_t k = 0; k < s.ncond; k++) {
             s.cop[k] = p->cond[u][k][0];
             s.cpos[k] = p->rev[p->cond[u][k][1]];
-            if (s.cop[k] > 2 || s.cpos[k] >= id) return set_err(c, G2_EINVAL, "bad condition");
+            if (s.cop[k] > 2 || s.cpos[k] >= id) return (*why = "bad condition", G2_EINVAL);
         }
         s.shared = p->shared[id] ? 1 : 0;
         if (s.shared) {
-            if (id < 2 || p->pre_bn_count[id] == 0) return set_err(c, G2_EINVAL, "shared position without pre back neighbours");
+            if (id < 2 || p->pre_bn_count[id] == 0) return (*why = "shared position without pre back neighbours", G2_EINVAL);
             s.npre = p->pre_bn_count[id];
             for (uint32_t k = 0; k < s.npre; k++) {
                 s.pre[k] = p->rev[p->pre_bn[id][k]];
-                if (s.pre[k] + 1 >= id) return set_err(c, G2_EINVAL, "pre back neighbour not in the prefix");
+                if (s.pre[k] + 1 >= id) return (*why = "pre back neighbour not in the prefix", G2_EINVAL);
             }
             s.nafter = p->after_bn_count[id];
             for (uint32_t k = 0; k < s.nafter; k++) {
                 s.after[k] = p->rev[p->after_bn[id][k]];
-                if (s.after[k] + 1 != id) return set_err(c, G2_EINVAL, "after back neighbour is not the previous position");
+                if (s.after[k] + 1 != id) return (*why = "after back neighbour is not the previous position", G2_EINVAL);
             }
             s.npcond = p->pre_cond_count[u];
             for (uint32_t k = 0; k < s.npcond; k++) {
                 s.pcop[k] = p->pre_cond[u][k][0];
                 s.pcpos[k] = p->rev[p->pre_cond[u][k][1]];
-                if (s.pcop[k] > 2 || s.pcpos[k] + 1 >= id) return set_err(c, G2_EINVAL, "bad pre condition");
+                if (s.pcop[k] > 2 || s.pcpos[k] + 1 >= id) return (*why = "bad pre condition", G2_EINVAL);
             }
             s.nacond = p->after_cond_count[u];
             for (uint32_t k = 0; k < s.nacond; k++) {
                 s.acop[k] = p->after_cond[u][k][0];
                 s.acpos[k] = p->rev[p->after_cond[u][k][1]];
-                if (s.acop[k] > 2 || s.acpos[k] + 1 != id) return set_err(c, G2_EINVAL, "bad after condition");
+                if (s.acop[k] > 2 || s.acpos[k] + 1 != id) return (*why = "bad after condition", G2_EINVAL);
             }
         }
         s.store_prefix = (id + 1 < n && p->strategy[id + 1] == 1) ? 1 : 0;
     }
-    if (g.step[1].shared) return set_err(c, G2_EINVAL, "position 1 cannot be shared");
+    if (g.step[1].shared) return (*why = "position 1 cannot be shared", G2_EINVAL);
     // A shared position whose pre back neighbours and pre conditions are, as
     // sets, the back neighbours and conditions of the previous (non-shared)
     // position filters exactly the same candidates: reuse that list.
@@ -561,6 +552,14 @@ int gm_compile(g2_ctx* c, const g2_plan* p) {
         }
         s.pre_is_prev = same ? 1 : 0;
     }
+    *out = g;
+    return G2_OK;
+}
+
+int gm_compile(g2_ctx* c, const g2_plan* p) {
+    GmProgram g{};
+    const char* why = "";
+    if (gm_program(p, &g, &why) != G2_OK) return set_err(c, G2_EINVAL, why);
     c->prog = g;
     c->plan = *p;
     c->has_plan = true;
@@ -570,14 +569,15 @@ int gm_compile(g2_ctx* c, const g2_plan* p) {
 
 int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
-    // whole-graph totals of a root-dominated plan: oriented local bitmaps
+    // totals of a root-dominated plan over the whole graph or a view whose
+    // sources' neighbours all have their rows: oriented local bitmaps
     // (per-root counts and the byte model follow the reference's traversal)
     // (it keeps row offsets in 32 bits and sorts ranks as int-sized lists: fewer
     // than 2^32 row entries and 2^31 - 1 vertices)
-    if (c->local_ok && c->sources_all && c->ne < (1ull << 32) && c->nv + 1 <= 0x7fffffffull && !per_root &&
+    if (c->local_ok && (c->sources_all || c->sources_closed) && c->ne < (1ull << 32) && c->nv + 1 <= 0x7fffffffull && !per_root &&
         !account && !(flags & G2_RUN_CANONICAL))
         return gm_local_run(c, count, st);
-    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
+    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kGmMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM) / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
     const uint64_t per_wave = cap * 2 * c->prog.n * 4;
@@ -692,7 +692,7 @@ int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* c
     const uint32_t rpt = std::max<uint32_t>(cfg->roots_per_task, 1);
     const uint64_t per_q = (ntasks + rpt - 1) / rpt;
     const uint32_t wpc = cfg->waves_per_cu ? cfg->waves_per_cu : waves_per_cu(c, WK_GM);
-    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
+    const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kGmMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * wpc / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);
     const uint64_t per_wave = cap * 2 * levels * 4;
@@ -766,6 +766,15 @@ int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* c
         st->max_scratch_words = cap;
     }
     return G2_OK;
+}
+
+}  // namespace g2
+
+namespace g2 {
+
+hipError_t gm_launch(const GmArgs& a, const GmProgram& prog, uint64_t blocks, uint32_t block_waves, hipStream_t s) {
+    hipLaunchKernelGGL(gm_kernel<false>, dim3(blocks), dim3(block_waves * kWave), 0, s, a, prog);
+    return hipGetLastError();
 }
 
 }  // namespace g2

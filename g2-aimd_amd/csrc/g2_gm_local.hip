@@ -2,9 +2,11 @@
 //
 // Applies when the plan's root (position 0) is a back neighbour of, and
 // ordered below (M(u_i) > M(u_0)), every other position -- every k-clique
-// plan among the reference's presets -- and the run covers every vertex as a
-// root.  Then each match lies in U(v0) = { w in N(v0) : w above v0 } and the
-// plan's back-neighbour / condition tables become bit operations over U.
+// plan among the reference's presets -- and every neighbour of a source has
+// its full row in the view (the whole graph, or a view bin materialised with
+// hop >= 2, view/view_bin.h:61-159).  Then each match lies in
+// U(v0) = { w in N(v0) : w above v0 } and the plan's back-neighbour /
+// condition tables become bit operations over U.
 //
 // Orientation: "above" compares the rank pi(v) of the key (degree, id)
 // instead of the id.  The plan's condition pairs are a complete symmetry
@@ -14,6 +16,15 @@
 // over all roots is the same number the reference's id order gives
 // (app_gmatch/gmatch.h:559-588), while |U| is bounded by O(sqrt(E)) instead
 // of by the largest degree.
+//
+// A view bin counts the matches whose lowest-ranked vertex is one of its
+// sources.  The ranks of a view are taken over its own degrees; a source and
+// its neighbours have their full rows, so their degrees and hence their
+// relative order are the whole graph's, and vertices without a row rank
+// below every vertex with one and never enter a U.  Summed over bins that
+// partition the vertices, the counts give the whole-graph total (the
+// reference's per-bin partial counts, which root each match at its
+// lowest-id vertex, are split differently; their sum is the same).
 //
 // Each run first builds the oriented relabelled CSR on the device:
 //   pi      = rank of (degree, id) over all vertices (radix sort),
@@ -760,8 +771,15 @@ __global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__
     }
 }
 
+// Ranks of the sources: the roots of a view whose sources are a subset.
+__global__ __launch_bounds__(256) void oroot_kernel(const uint32_t* __restrict__ src, uint64_t ns,
+                                                    const uint32_t* __restrict__ pi, uint32_t* __restrict__ roots) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (uint64_t)gridDim.x * blockDim.x)
+        roots[i] = pi[src[i]];
+}
+
 // Build pi, orp and ocols of the uploaded view on the context's stream.
-static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocols) {
+static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocols, const uint32_t** rank) {
     const uint64_t nv = c->nv, ne = c->ne;
     // keys (degree << 32 | id): the degree needs bitlen(max degree) bits above the id
     const int nbits = (int)(32 + std::max<uint64_t>(1, 64 - __builtin_clzll(std::max<uint64_t>(c->max_degree, 1))));
@@ -807,6 +825,7 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
     if ((e = hipGetLastError())) return hip_err(c, e, "oriented rows kernels");
     *orp = (const uint64_t*)d_orp;
     *ocols = d_ocols;
+    *rank = pi;
     return G2_OK;
 }
 
@@ -915,10 +934,16 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if (c->nv + 1 > 0x7fffffffull) return set_err(c, G2_EINVAL, "local-bitmap matching supports fewer than 2^31-1 vertices");
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kLocalBlockWaves);
     const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM_LOCAL) / bw);
+    // roots: every rank of a whole-graph view, else the ranks of the sources
+    // (a clique is then counted in the view holding its lowest-ranked vertex)
+    const bool all = c->sources_all;
+    const uint64_t nroots = all ? c->nv : c->ns;
+    const uint64_t list_len = std::max<uint64_t>(nroots, 1);
     int rc;
     if ((rc = ensure(c, c->counters, LCTR_N * 8, "counters"))) return rc;
-    if ((rc = ensure(c, c->big_list2, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
-    if ((rc = ensure(c, c->list, std::max<uint64_t>(c->nv, 1) * 4, "deferred roots"))) return rc;
+    if ((rc = ensure(c, c->big_list2, list_len * 4, "deferred roots"))) return rc;
+    if ((rc = ensure(c, c->list, list_len * 4, "deferred roots"))) return rc;
+    if (!all && (rc = ensure(c, c->roots, list_len * 4, "root ranks"))) return rc;
     hipError_t e;
     auto* ctr = (unsigned long long*)c->counters.ptr;
     uint32_t* lists[2] = {(uint32_t*)c->big_list2.ptr, (uint32_t*)c->list.ptr};
@@ -927,7 +952,14 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if ((e = hipEventRecord(c->ev[0], c->stream))) return hip_err(c, e, "event");
     const uint64_t* orp = nullptr;
     const uint32_t* ocols = nullptr;
-    if ((rc = build_oriented(c, &orp, &ocols))) return rc;
+    const uint32_t* pi = nullptr;
+    if ((rc = build_oriented(c, &orp, &ocols, &pi))) return rc;
+    if (!all && c->ns) {
+        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((c->ns + 255) / 256, 8192));
+        hipLaunchKernelGGL(oroot_kernel, dim3(grid), dim3(256), 0, c->stream, (const uint32_t*)c->src.ptr, c->ns, pi,
+                           (uint32_t*)c->roots.ptr);
+        if ((e = hipGetLastError())) return hip_err(c, e, "root ranks");
+    }
     if ((e = hipMemsetAsync(ctr, 0, LCTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
     LocalArgs a{};
     a.orp = orp;
@@ -960,8 +992,8 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         return G2_OK;
     };
     // pass 1: every root, rows in the wave's LDS
-    a.list = nullptr;
-    a.ntasks = c->nv;
+    a.list = all ? nullptr : (const uint32_t*)c->roots.ptr;
+    a.ntasks = nroots;
     a.scratch = nullptr;
     a.scratch_words = 0;
     a.big_list = lists[0];
@@ -970,6 +1002,7 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     uint32_t launches = 1;
     const uint64_t n1 = h[LCTR_DEFER];
     uint64_t n = n1, need = h[LCTR_DEFER + 1];
+    uint64_t pass_roots[5] = {nroots - n1, 0, 0, 0, 0};
     int in = 0;  // list holding the roots of the next pass
     const struct { LocalPass pass; int per_cu; } groups[3] = {
         {LocalPass::Group256, kB1PerCu}, {LocalPass::Group512, kB2PerCu}, {LocalPass::Group1024, kB3PerCu}};
@@ -980,6 +1013,7 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         a.defer_ctr = LCTR_DEFER + 2 * (g + 1);
         if ((rc = timed(groups[g].pass, std::min<uint64_t>((uint64_t)c->num_cus * groups[g].per_cu, n)))) return rc;
         launches++;
+        pass_roots[g + 1] = n - h[a.defer_ctr];
         n = h[a.defer_ctr];
         need = h[a.defer_ctr + 1];
         in ^= 1;
@@ -996,6 +1030,7 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         a.big_list = nullptr;
         if ((rc = timed(LocalPass::Wave, bwaves / bw))) return rc;
         launches++;
+        pass_roots[4] = n;
     }
     if ((e = hipEventRecord(c->ev[3], c->stream))) return hip_err(c, e, "event");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
@@ -1010,6 +1045,10 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         st->alg_bytes = 0;
         st->big_roots = n1;
         st->max_scratch_words = need;
+        st->wave_idle_frac = 0;
+        st->donations = 0;
+        st->local_bitmap = 1;
+        for (int k = 0; k < 5; k++) st->local_pass_roots[k] = pass_roots[k];
     }
     return G2_OK;
 }

@@ -38,6 +38,24 @@ struct GmProgram {
     GmStep step[G2_MAXQ];
 };
 
+// canonical matching kernel: counters, then its arguments
+enum { GCTR_NEXT = 0, GCTR_COUNT, GCTR_BYTES, GCTR_NODES, GCTR_N };
+constexpr int kGmMaxBlockWaves = 4;
+
+struct GmArgs {
+    const uint64_t* __restrict__ rp;
+    const uint32_t* __restrict__ cols;
+    const uint32_t* __restrict__ src;
+    const uint64_t* __restrict__ task_off;  // ns+1 window offsets per source (null: one task per source)
+    uint64_t ns;
+    uint64_t ntasks;
+    uint32_t chunk;                         // level-1 window length (0 = whole row)
+    uint32_t* scratch;
+    uint64_t cap;                           // entries per list
+    unsigned long long* ctr;
+    unsigned long long* per_root;           // nullable
+};
+
 // Oriented local-bitmap form of a plan whose root is adjacent to, and ordered
 // below, every other query vertex (all k-cliques among the presets).  Every
 // table is by position >= 1; bn_mask excludes the root.
@@ -65,7 +83,9 @@ struct g2_ctx {
     g2::DevBuf rp, cols, src;
     uint64_t nv = 0, ne = 0, ns = 0;
     uint64_t max_degree = 0;
+    bool view_ok = false;         // the last upload completed (runs need a view)
     bool sources_all = false;     // sources == [0, nv) in order (whole-graph view)
+    bool sources_closed = false;  // every neighbour of a source has its row in the view
     g2::DevBuf deg;               // u32 degree of every vertex
     std::vector<uint64_t> h_gm_task_off;  // host copy: level-1 window offsets per source
     g2::DevBuf gm_task_off;
@@ -90,6 +110,7 @@ struct g2_ctx {
     g2::LocalProgram lprog{};
     g2::DevBuf big_list2;
     g2::DevBuf orient;            // oriented relabelled rows of the local-bitmap path
+    g2::DevBuf roots;             // local-bitmap path: ranks of the sources of a partial view
     g2_plan plan{};
 
     // batched matching
@@ -121,6 +142,10 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
             uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st);
 int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_count, g2_stats* st);
 int gm_compile(g2_ctx* c, const g2_plan* p);
+// plan tables re-indexed by position; G2_EINVAL with *why naming the first broken table
+int gm_program(const g2_plan* p, GmProgram* out, const char** why);
+// gm_kernel (no byte accounting) over `blocks` workgroups of block_waves waves
+hipError_t gm_launch(const GmArgs& a, const GmProgram& prog, uint64_t blocks, uint32_t block_waves, hipStream_t s);
 void gm_local_compile(g2_ctx* c, const g2_plan* p);
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st);
 int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,

@@ -436,7 +436,7 @@ extern "C" {
 
 int g2_features(g2_ctx* c, int64_t* out) {
     if (!c || !out) return G2_EINVAL;
-    if (!c->rp.ptr) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     return features_run(c, out);

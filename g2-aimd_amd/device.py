@@ -44,10 +44,13 @@ class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("launches", C.c_uint32), ("total_ms", C.c_double),
                 ("nodes", C.c_uint64), ("alg_bytes", C.c_uint64), ("big_roots", C.c_uint64),
                 ("max_scratch_words", C.c_uint64), ("wave_idle_frac", C.c_double),
-                ("donations", C.c_uint64)]
+                ("donations", C.c_uint64), ("local_bitmap", C.c_uint32), ("reserved", C.c_uint32),
+                ("local_pass_roots", C.c_uint64 * 5)]
 
     def to_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d["local_pass_roots"] = list(self.local_pass_roots)
+        return d
 
 
 class Sched(C.Structure):

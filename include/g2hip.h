@@ -73,6 +73,11 @@ typedef struct g2_stats {
     double   wave_idle_frac;      /* MCE: 1 - sum of per-wave busy spans / (waves x makespan),
                                      from the GPU real-time clock (tail idleness) */
     uint64_t donations;           /* MCE: search frames handed from busy to idle waves */
+    uint32_t local_bitmap;        /* matching: 1 if the oriented local-bitmap passes counted the run */
+    uint32_t reserved;
+    uint64_t local_pass_roots[5]; /* local-bitmap matching: roots each pass took to the end (rows in
+                                     a wave's LDS; in a 256-, 512-, 1024-thread workgroup's LDS;
+                                     in per-wave global arenas) */
 } g2_stats;
 
 /* root schedule for g2_mce_run (NULL = the given source order, one wave per root) */

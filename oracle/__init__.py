@@ -89,6 +89,8 @@ def bind(path: str) -> C.CDLL:
     L.orc_automorphism_count.argtypes = [C.c_uint32, u32p]
     L.orc_automorphism_count.restype = C.c_uint64
     L.orc_kclique_count.argtypes = [u64p, C.c_uint64, u32p, C.c_uint32, C.c_int, P64]
+    L.orc_kclique_count_ranked.argtypes = [u64p, C.c_uint64, u32p, C.c_uint32, u32p, C.c_uint64,
+                                           C.c_int, P64]
     return L
 
 
@@ -162,4 +164,14 @@ def kclique_count(g, k: int, threads: int = 0) -> int:
     out = C.c_uint64()
     rc = lib().orc_kclique_count(g.row_ptrs, g.nv, _cols(g), k, threads, C.byref(out))
     assert rc == 0
+    return out.value
+
+
+def kclique_count_ranked(g, k: int, sources=None, threads: int = 0) -> int:
+    """k-cliques whose lowest vertex under (degree, id) is a source."""
+    src = _src(sources, g.nv)
+    s = src if len(src) else np.zeros(1, np.uint32)
+    out = C.c_uint64()
+    rc = lib().orc_kclique_count_ranked(g.row_ptrs, g.nv, _cols(g), k, s, len(src), threads, C.byref(out))
+    assert rc == 0, rc
     return out.value

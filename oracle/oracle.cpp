@@ -535,4 +535,53 @@ int orc_kclique_count(const uint64_t* rp, uint64_t nv, const uint32_t* cols, uin
     return 0;
 }
 
+// k-cliques whose lowest vertex under the key (degree, id) is one of the given
+// sources, k in [2,8]: how a view bin's count is split off the whole-graph
+// total when each clique is counted at its lowest-degree vertex (ties to the
+// lower id) instead of at its lowest id (app_gmatch/gmatch.h:572-578 with the
+// plan's conditions).  Over sources that partition the vertices the counts
+// add up to orc_kclique_count's total.  Degrees are the rows' lengths in the
+// CSR given, as the view holds them.
+int orc_kclique_count_ranked(const uint64_t* rp, uint64_t nv, const uint32_t* cols, uint32_t k,
+                             const uint32_t* sources, uint64_t ns, int nthreads, uint64_t* out) {
+    if (k < 2 || k > 8) return -1;
+    for (uint64_t i = 0; i < ns; i++)
+        if (sources[i] >= nv) return -2;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    auto above = [&](uint32_t a, uint32_t b) {  // key(b) > key(a)
+        const uint64_t da = rp[a + 1] - rp[a], db = rp[b + 1] - rp[b];
+        return db > da || (db == da && b > a);
+    };
+    uint64_t tot = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+:tot)
+    {
+        std::vector<std::vector<uint32_t>> lvl(k + 1);
+        // r-cliques inside the sorted candidate set lvl[d], each taken in id order
+        auto count = [&](auto&& self, uint32_t d, uint32_t r) -> uint64_t {
+            const std::vector<uint32_t>& C = lvl[d];
+            if (r == 1) return C.size();
+            uint64_t n = 0;
+            for (size_t i = 0; i < C.size(); i++) {
+                std::vector<uint32_t>& nx = lvl[d + 1];
+                nx.clear();
+                const uint32_t c = C[i];
+                for (size_t j = i + 1; j < C.size(); j++)
+                    if (member(cols, rp[c], rp[c + 1], C[j])) nx.push_back(C[j]);
+                n += self(self, d + 1, r - 1);
+            }
+            return n;
+        };
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < (int64_t)ns; i++) {
+            const uint32_t s = sources[i];
+            lvl[0].clear();
+            for (uint64_t e = rp[s]; e < rp[s + 1]; e++)
+                if (above(s, cols[e])) lvl[0].push_back(cols[e]);
+            tot += count(count, 0, k - 1);
+        }
+    }
+    *out = tot;
+    return 0;
+}
+
 }  // extern "C"

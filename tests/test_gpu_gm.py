@@ -64,11 +64,16 @@ def test_clique_closed_forms(g2, orc, gpu_exec):
 
 
 def test_triangles_4cliques_rmat15(g2, orc, gpu_exec):
+    """Canonical traversal (per-root counts and byte accounting) and the
+    local-bitmap path (whole-graph totals) on the same graph."""
     print("case: rmat-15 q0 q24", flush=True)
     g = g2.gen_rmat(15, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
     for q in (0, 24):
         _, qadj = g2.query_preset(q)
-        both(g2, orc, gpu_exec, g, qadj, account=True)
+        got = both(g2, orc, gpu_exec, g, qadj, account=True)
+        fast = gpu_exec.run_gm()
+        assert fast["stats"]["local_bitmap"] == 1
+        assert fast["count"] == got["count"]
 
 
 def test_windows_and_views(g2, orc, rmat12):

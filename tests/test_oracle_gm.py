@@ -5,6 +5,7 @@ import json
 import math
 import os
 
+import numpy as np
 import pytest
 
 import graphs as G
@@ -98,3 +99,26 @@ def test_byte_model_hand_worked(g2, orc):
     assert list(p.strategy)[:3] == [0, 0, 0] and list(p.bn[2])[:2] == [0, 1]
     r = orc.gm(G.complete(g2, 4), p)
     assert r["count"] == 4 and r["bytes"] == 140 + 88 + 48 + 20
+
+
+def test_ranked_kclique_split(g2, orc):
+    """orc_kclique_count_ranked: every source set of a partition counts the
+    cliques whose lowest (degree, id) vertex it holds, so the parts add up to
+    the whole-graph count of the id-ordered counter and of orc_gm; all
+    sources give the whole count; a hand-worked case fixes the rule."""
+    g = g2.gen_rmat(10, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000002).preprocess()
+    bins = np.random.default_rng(1).integers(0, 4, g.nv)
+    for q, k in ((0, 3), (24, 4), (25, 5)):
+        whole = orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(g2.plan_for_preset(q))))["count"]
+        assert orc.kclique_count(g, k) == whole
+        assert orc.kclique_count_ranked(g, k) == whole
+        parts = [orc.kclique_count_ranked(g, k, sources=np.nonzero(bins == b)[0].astype(np.uint32))
+                 for b in range(4)]
+        assert sum(parts) == whole
+    # triangle {0,1,2} plus the edge 2-3: degrees 2,2,3,1.  The lowest key of
+    # the triangle is vertex 0 (degree 2, lower id than 1)
+    tri = G.from_pairs(g2, [(0, 1), (0, 2), (1, 2), (2, 3)], 4)
+    assert [orc.kclique_count_ranked(tri, 3, sources=np.array([v], np.uint32)) for v in range(4)] == [1, 0, 0, 0]
+    # relabelled so that the lowest id is the degree-3 vertex: the rule follows degree first
+    tri2 = G.from_pairs(g2, [(1, 2), (1, 0), (2, 0), (0, 3)], 4)
+    assert [orc.kclique_count_ranked(tri2, 3, sources=np.array([v], np.uint32)) for v in range(4)] == [0, 1, 0, 0]
