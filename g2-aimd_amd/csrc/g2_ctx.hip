@@ -130,6 +130,7 @@ void g2_destroy(g2_ctx* c) {
                       &c->big_scratch, &c->list, &c->hist, &c->per_root, &c->order, &c->deg, &c->big_list2, &c->spans,
                       &c->batch_off, &c->batch_progs, &c->batch_q, &c->donate, &c->taskbuild, &c->orient, &c->roots})
         release(c, *b);
+    if (c->spill) (void)hipHostFree(c->spill);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -238,7 +239,9 @@ int g2_mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     if (st) *st = g2_stats{};
-    return mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);
+    const int rc = mce_run(c, sched, flags, total, hist, hist_len, per_root_nodes, st);
+    if (st && rc == G2_OK) st->device_bytes = c->used;
+    return rc;
 }
 
 int g2_gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
@@ -249,7 +252,9 @@ int g2_gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     if (st) *st = g2_stats{};
-    return gm_run_batch(c, plans, nq, cfg, counts, nodes, over_budget, st);
+    const int rc = gm_run_batch(c, plans, nq, cfg, counts, nodes, over_budget, st);
+    if (st && rc == G2_OK) st->device_bytes = c->used;
+    return rc;
 }
 
 int g2_plan_local_eligible(const g2_plan* p) {
@@ -271,7 +276,9 @@ int g2_gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root_cou
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     if (st) *st = g2_stats{};
-    return gm_run(c, flags, count, per_root_count, st);
+    const int rc = gm_run(c, flags, count, per_root_count, st);
+    if (st && rc == G2_OK) st->device_bytes = c->used;
+    return rc;
 }
 
 }  // extern "C"

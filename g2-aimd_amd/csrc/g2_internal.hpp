@@ -102,6 +102,8 @@ struct g2_ctx {
     g2::DevBuf spans;
     g2::DevBuf donate;            // MCE donation queue (counters, record table, pool)
     g2::DevBuf taskbuild;         // MCE task-list construction scratch
+    void* spill = nullptr;        // MCE host spill: page-locked large-root arenas
+    uint64_t spill_bytes = 0;
 
     // matching
     bool has_plan = false;

@@ -1352,14 +1352,40 @@ static int build_tasks(g2_ctx* c, const g2_sched* sched, uint32_t G, uint64_t* n
     return G2_OK;
 }
 
+// page-locked host arenas of the large-root pass (grow-only, kept with the
+// context, not charged to the device budget)
+static int host_spill(g2_ctx* c, uint64_t bytes) {
+    if (bytes <= c->spill_bytes) return G2_OK;
+    if (c->spill) {
+        hipError_t e = hipHostFree(c->spill);
+        c->spill = nullptr;
+        c->spill_bytes = 0;
+        if (e != hipSuccess) return hip_err(c, e, "hipHostFree (host spill)");
+    }
+    hipError_t e = hipHostMalloc(&c->spill, bytes, hipHostMallocNonCoherent);
+    if (e != hipSuccess) {
+        c->spill = nullptr;
+        return hip_err(c, e, "hipHostMalloc (host spill of the large-root arenas)");
+    }
+    c->spill_bytes = bytes;
+    return G2_OK;
+}
+
 int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, uint64_t* hist,
             uint32_t hist_len, uint64_t* per_root_nodes, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
     const uint32_t hl = hist ? hist_len : 2;
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kMaxBlockWaves);
-    const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_MCE) / bw);
-    const uint64_t nwaves = nblocks * bw;
+    uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_MCE) / bw);
     const uint64_t sw = c->tune.wave_scratch_words;
+    // under a budget the grid shrinks until its wave arenas fit what the
+    // budget leaves (fewer resident waves, same result)
+    if (c->budget) {
+        const uint64_t other = c->used - c->scratch.bytes;
+        const uint64_t room = c->budget > other ? c->budget - other : 0;
+        while (nblocks > 1 && nblocks * bw * sw * 8 > room) nblocks /= 2;
+    }
+    const uint64_t nwaves = nblocks * bw;
     int rc;
     if ((rc = ensure(c, c->counters, CTR_N * 8, "counters"))) return rc;
     if ((rc = ensure(c, c->hist, (uint64_t)hl * 8, "histogram"))) return rc;
@@ -1373,13 +1399,17 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     const uint32_t rpw = c->tune.mce_roots_per_wave;
     const uint32_t G = account ? 0u : rpw >= 4 ? 16u : rpw >= 2 ? 32u : 0u;
     uint64_t nsmall = 0;  // roots for the lane-group kernel, after the main list
-    if (G || (sched && (sched->order || sched->split || sched->end > sched->begin))) {
+    if (G || (sched && (sched->order || sched->split))) {
         uint64_t nmain = 0;
         if ((rc = build_tasks(c, sched, G, &nmain, &nsmall))) return rc;
         tasks = (const uint64_t*)c->order.ptr;
         ntasks = nmain;
+    } else if (sched && sched->end > sched->begin) {
+        // a range of the source order needs no task list: tasks [begin, end)
+        first = sched->begin;
+        ntasks = sched->end;
     }
-    if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks, 1) * 8, "deferred roots"))) return rc;
+    if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks - first, 1) * 8, "deferred roots"))) return rc;
     // donation queue: counters, record table and a word pool (a record is a frame plus two words)
     const uint32_t every = account ? 0u : (c->tune.mce_donate_every == 0xffffffffu ? 0u : c->tune.mce_donate_every);
     const uint64_t max_records = 1ull << 16, pool_words = 1ull << 23;
@@ -1425,11 +1455,14 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
     a.max_records = max_records;
     a.donate_every = every;
 
-    auto launch = [&](uint64_t blocks) {
-        if (account) hipLaunchKernelGGL(mce_kernel<true>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);
-        else hipLaunchKernelGGL(mce_kernel<false>, dim3(blocks), dim3(bw * kWave), 0, c->stream, a);
+    auto launch_shape = [&](uint64_t blocks, uint32_t waves) {
+        if (account) hipLaunchKernelGGL(mce_kernel<true>, dim3(blocks), dim3(waves * kWave), 0, c->stream, a);
+        else hipLaunchKernelGGL(mce_kernel<false>, dim3(blocks), dim3(waves * kWave), 0, c->stream, a);
         return hipGetLastError();
     };
+    auto launch = [&](uint64_t blocks) { return launch_shape(blocks, bw); };
+    uint64_t big_blocks = 0;
+    uint32_t big_bw = bw;
     float k1 = 0, k2 = 0, tot = 0;
     uint32_t launches = 1;
     if ((e = reset_queue(nwaves))) return hip_err(c, e, "donation queue reset");
@@ -1459,24 +1492,48 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         return hip_err(c, e, "D2H donation counters");
     uint64_t donations = h_made;
     const uint64_t nbig = h_ctr[CTR_NBIG];
+    uint64_t spill_bytes = 0;
     if (nbig) {
-        // large-root pass: arenas sized for the largest deferred root
+        // large-root pass: arenas sized for the largest deferred root, as
+        // many waves as the budget holds (whole workgroups of bw waves, or
+        // one smaller workgroup)
         const uint64_t need = h_ctr[CTR_MAXNEED];
         uint64_t bwaves = std::min<uint64_t>(c->tune.big_waves, nbig);
         while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
-        bwaves = (bwaves + bw - 1) / bw * bw;
-        if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas"))) return rc;
+        big_bw = (uint32_t)std::min<uint64_t>(bw, bwaves);
+        bwaves = bwaves / big_bw * big_bw;
+        uint64_t* arenas = nullptr;
+        if ((rc = ensure(c, c->big_scratch, bwaves * need * 8, "large-root arenas")) == G2_OK) {
+            arenas = (uint64_t*)c->big_scratch.ptr;
+        } else if (rc == G2_EBUDGET) {
+            // Host spill: the device budget cannot hold even one wave's arena,
+            // so the arenas go to page-locked host memory that the kernel
+            // reads and writes over PCIe (coarse-grained: each arena is used
+            // by one wave only, its atomics stay in that XCD's L2).  Slow, and
+            // only for budgets far below the roots' needs; the result is the
+            // same.  The reference's host spill is dumpToHost / loadFromHost
+            // (app_BK/BK.h:357-398, system/subgraph_container.h:213-216).
+            bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), nbig);
+            big_bw = (uint32_t)std::min<uint64_t>(bw, bwaves);
+            bwaves = bwaves / big_bw * big_bw;
+            if ((rc = host_spill(c, bwaves * need * 8))) return rc;
+            arenas = (uint64_t*)c->spill;
+            spill_bytes = bwaves * need * 8;
+        } else {
+            return rc;
+        }
+        big_blocks = bwaves / big_bw;
         a.tasks = (const uint64_t*)c->list.ptr;
         a.first = 0;
         a.ntasks = nbig;
-        a.scratch = (uint64_t*)c->big_scratch.ptr;
+        a.scratch = arenas;
         a.scratch_words = need;
         a.big_list = nullptr;
         a.wave_span = nullptr;
         if ((e = hipMemsetAsync(ctr + CTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
         if ((e = reset_queue(bwaves))) return hip_err(c, e, "donation queue reset");
         if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
-        if ((e = launch(bwaves / bw))) return hip_err(c, e, "mce large-root launch");
+        if ((e = launch_shape(big_blocks, big_bw))) return hip_err(c, e, "mce large-root launch");
         if ((e = hipEventRecord(c->ev[2], c->stream))) return hip_err(c, e, "event");
         launches = 2;
     }
@@ -1532,6 +1589,7 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         st->max_scratch_words = h_ctr[CTR_MAXNEED];
         st->wave_idle_frac = idle;
         st->donations = donations;
+        st->host_spill_bytes = spill_bytes;
     }
     return G2_OK;
 }

@@ -45,7 +45,8 @@ class Stats(C.Structure):
                 ("nodes", C.c_uint64), ("alg_bytes", C.c_uint64), ("big_roots", C.c_uint64),
                 ("max_scratch_words", C.c_uint64), ("wave_idle_frac", C.c_double),
                 ("donations", C.c_uint64), ("local_bitmap", C.c_uint32), ("reserved", C.c_uint32),
-                ("local_pass_roots", C.c_uint64 * 5)]
+                ("local_pass_roots", C.c_uint64 * 5), ("host_spill_bytes", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
 
     def to_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

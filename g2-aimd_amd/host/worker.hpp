@@ -15,7 +15,25 @@
 //     memory per executor, system/appbase.h:93-114);
 //   * `bk` prints the summed maximal-clique count as "Total counts" (the
 //     reference's device code never writes the counters that line sums, so it
-//     prints 0 there) and the clique-size histogram.
+//     prints 0 there) and the clique-size histogram;
+//   * a view's roots run in one launch; the reference's AIMD chunk loop
+//     (system/pipeline_executor.h:151-207) starts only after a launch does
+//     not fit the memory budget.
+// Additions (no reference counterpart):
+//   -gen social:<n>[:seed] | rmat:<scale>[:seed] | chunglu:<n>[:seed]
+//        a seeded synthetic graph instead of -dg, degeneracy-recoded and cut
+//        into -nb (default 4) kcore bins in memory (no files)
+//   -sched plain | learned | both     (bk) root schedule of every view:
+//        plain = source order, one root per wave, no work donation;
+//        learned = cost-model order and heavy-root splits, work donation,
+//        small roots batched several per wave (-rpw), -wpc resident waves
+//        per CU; both = the pipeline twice, plain then learned
+//   -model <file>   trained cost model (standardiser + MLP parameters)
+//   -train <file>   train the cost model first, on plain runs of the
+//        -train-gen graphs (comma-separated specs), and write it to <file>
+//   -epochs <n>     training epochs (20)
+//   -budget <GB>    device memory per consumer
+//   -json <file>    totals, histogram and per-form timings as JSON
 #pragma once
 #include <atomic>
 #include <condition_variable>
@@ -157,24 +175,61 @@ void reorder_bins(std::vector<ViewBin>& pool, const uint64_t* rp);
 struct RunResult {
     uint64_t count = 0;                 // GM matches / BK maximal cliques of this view
     std::vector<uint64_t> hist;         // BK clique sizes
+    double kernel_ms = 0;               // HIP-event time of the search kernels
+    double sched_ms = 0;                // learned schedule: features, inference, order
+    double idle_sum = 0;                // sum of per-launch wave idle fractions
+    uint32_t launches = 0;              // g2_mce_run / g2_gm_run calls that counted
+    uint32_t retries = 0;               // launches refused by the memory budget
+    uint64_t nodes = 0, donations = 0, split_roots = 0;
+    uint64_t host_spill_bytes = 0;      // largest host spill of a launch
+    uint64_t device_bytes = 0;          // largest device memory held after a launch
+    void add(const RunResult& r);
+};
+
+// cost model shared by the consumers of a learned run
+struct Model {
+    g2h_norm norm{};
+    std::vector<float> params;
+};
+
+// how a consumer schedules the roots of its views
+struct BkForm {
+    bool learned = false;
+    const Model* model = nullptr;       // learned form only
+    const uint32_t* core = nullptr;     // core numbers of the whole graph (learned form)
+    uint64_t nv = 0;
+    g2_tuning tune{};
 };
 
 class Executor {
    public:
-    Executor(int consumer, int device, uint64_t budget);
+    Executor(int consumer, int device, uint64_t budget, const g2_tuning& tune = g2_tuning{});
     ~Executor();
     void transfer(const Holder& h);
-    RunResult run_bk(uint32_t hist_len);
+    RunResult run_bk(uint32_t hist_len, const BkForm& form);
     RunResult run_gm();
     void set_plan(const g2_plan& p);
 
    private:
     void check(int rc, const char* what);
     int consumer_;
+    int device_;
     std::string tag_;
     g2_ctx* ctx_ = nullptr;
+    g2_mlp* mlp_ = nullptr;
     uint64_t ns_ = 0;
+    std::vector<uint32_t> sources_;
 };
+
+// synthetic input of -gen / -train-gen: preprocessed, degeneracy-recoded,
+// shell offsets for the kcore bins
+struct GenGraph {
+    g2h_graph* g = nullptr;
+    std::vector<uint64_t> shells;
+};
+GenGraph generate(const std::string& spec);
+// train the cost model on plain runs of the given graphs (specs) and return it
+Model train_model(const std::vector<std::string>& specs, int device, uint32_t epochs);
 
 // ---- the worker ------------------------------------------------------------
 enum class App { BK, GM };

@@ -12,10 +12,49 @@ idle at the end of a launch.  The scheduler
 Neither step changes a result: the kernel's tree, counts, histogram and
 per-root node totals are the same for every order and split (tests).  The
 plain source order with no split is the fallback and the yardstick.
+
+The feature assembly, standardisation, schedule rule, core numbers and model
+files are C++ in libg2host.so (host/sched.cpp, include/g2host.h), shared with
+the `bk` command line; this module binds them for the tests and bench.py.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
+
+from . import host_lib
+
+NFEAT = 8
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+class Norm(C.Structure):
+    """g2h_norm: per-feature mean and scale"""
+    _fields_ = [("mu", C.c_float * NFEAT), ("sd", C.c_float * NFEAT)]
+
+
+def _lib():
+    L = host_lib()
+    if not getattr(L, "_sched_bound", False):
+        L.g2h_core_numbers.argtypes = [_u64p, C.c_uint64, _u32p, _u32p]
+        L.g2h_root_features.argtypes = [_i64p, _u32p, C.c_uint64, _u32p, C.c_uint64, _f32p]
+        L.g2h_norm_fit.argtypes = [_f32p, C.c_uint64, C.POINTER(Norm)]
+        L.g2h_norm_apply.argtypes = [C.POINTER(Norm), _f32p, C.c_uint64]
+        L.g2h_norm_apply.restype = None
+        L.g2h_learned_schedule.argtypes = [_f32p, _i64p, C.c_uint64, C.c_uint64, C.c_uint32, _u32p, _u32p]
+        L.g2h_model_save.argtypes = [C.c_char_p, C.POINTER(Norm), _f32p, C.c_uint64]
+        L.g2h_model_load.argtypes = [C.c_char_p, C.POINTER(Norm), _f32p, C.c_uint64]
+        L._sched_bound = True
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {host_lib().g2h_last_error().decode()}")
 
 FEATURE_NAMES = ("log1p_degree", "log1p_core", "log1p_2paths", "log1p_triangles", "clustering",
                  "degeneracy_rank", "shared_neighbour_fraction", "log1p_later_neighbours")
@@ -29,16 +68,23 @@ def core_numbers(shells: np.ndarray, nv: int) -> np.ndarray:
     return core
 
 
+def core_numbers_csr(g) -> np.ndarray:
+    """core number of every vertex of any CSR graph (bucket peel, C++)"""
+    out = np.zeros(max(g.nv, 1), np.uint32)
+    cols = g.cols if g.ne else np.zeros(1, np.uint32)
+    _check(_lib().g2h_core_numbers(g.row_ptrs, g.nv, cols, out), "g2h_core_numbers")
+    return out[: g.nv]
+
+
 def root_features(ex, sources: np.ndarray, core: np.ndarray, nv: int) -> np.ndarray:
     """float32 (ns, 8) model inputs of the uploaded sources"""
-    f = ex.features().astype(np.float64)
-    deg, two, tri, shared, p0 = (f[:, i] for i in range(5))
-    pairs = deg * (deg - 1)
-    clust = np.where(pairs > 0, 2 * tri / np.maximum(pairs, 1), 0.0)
-    src = np.asarray(sources, dtype=np.int64)
-    x = np.stack([np.log1p(deg), np.log1p(core[src]), np.log1p(two), np.log1p(tri), clust,
-                  src / max(nv, 1), shared / np.maximum(deg, 1), np.log1p(p0)], axis=1)
-    return x.astype(np.float32)
+    f = np.ascontiguousarray(ex.features(), dtype=np.int64)
+    src = np.ascontiguousarray(sources, dtype=np.uint32)
+    cr = np.ascontiguousarray(core, dtype=np.uint32)
+    out = np.zeros((max(len(src), 1), NFEAT), np.float32)
+    if len(src):
+        _check(_lib().g2h_root_features(f, src, len(src), cr, nv, out), "g2h_root_features")
+    return out[: len(src)]
 
 
 def learned_schedule(pred_log_nodes: np.ndarray, later_neighbours: np.ndarray, waves: int,
@@ -47,13 +93,15 @@ def learned_schedule(pred_log_nodes: np.ndarray, later_neighbours: np.ndarray, w
     a root predicted to exceed the per-wave share is dealt into
     ceil(cost / share) parts (at most its number of later neighbours, the
     most first-level children it can have)."""
-    cost = np.expm1(np.clip(pred_log_nodes.astype(np.float64), 0, 60))
-    order = np.argsort(-cost, kind="stable").astype(np.uint32)
-    share = max(cost.sum() / max(waves, 1), 1.0)
-    split = np.ceil(cost / share)
-    split = np.minimum(split, np.maximum(later_neighbours, 1))
-    split = np.clip(split, 1, max_split).astype(np.uint32)
-    return order, split
+    pred = np.ascontiguousarray(pred_log_nodes, dtype=np.float32)
+    later = np.ascontiguousarray(later_neighbours, dtype=np.int64)
+    n = len(pred)
+    order = np.zeros(max(n, 1), np.uint32)
+    split = np.zeros(max(n, 1), np.uint32)
+    _check(_lib().g2h_learned_schedule(pred if n else np.zeros(1, np.float32),
+                                       later if n else np.zeros(1, np.int64), n, waves, max_split,
+                                       order, split), "g2h_learned_schedule")
+    return order[:n], split[:n]
 
 
 def spearman(a: np.ndarray, b: np.ndarray) -> float:
@@ -66,14 +114,39 @@ def spearman(a: np.ndarray, b: np.ndarray) -> float:
 
 
 class Standardizer:
-    """per-feature affine map fitted on the training rows"""
+    """per-feature affine map fitted on the training rows (g2h_norm_fit)"""
 
-    def __init__(self, x: np.ndarray):
-        self.mu = x.mean(axis=0)
-        self.sd = np.maximum(x.std(axis=0), 1e-6)
+    def __init__(self, x=None, norm: Norm | None = None):
+        self.norm = Norm() if norm is None else norm
+        if x is not None:
+            xx = np.ascontiguousarray(x, dtype=np.float32)
+            _check(_lib().g2h_norm_fit(xx, len(xx), C.byref(self.norm)), "g2h_norm_fit")
+
+    @property
+    def mu(self):
+        return np.array(self.norm.mu[:], np.float32)
+
+    @property
+    def sd(self):
+        return np.array(self.norm.sd[:], np.float32)
 
     def __call__(self, x: np.ndarray) -> np.ndarray:
-        return ((x - self.mu) / self.sd).astype(np.float32)
+        out = np.array(x, dtype=np.float32, order="C", copy=True)
+        _lib().g2h_norm_apply(C.byref(self.norm), out, len(out))
+        return out
+
+
+def save_model(path: str, norm: Standardizer, params: np.ndarray) -> None:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    _check(_lib().g2h_model_save(path.encode(), C.byref(norm.norm), p, len(p)), "g2h_model_save")
+
+
+def load_model(path: str, nparams: int):
+    """(Standardizer, parameters) of a model file written by save_model / bk"""
+    n = Norm()
+    p = np.zeros(nparams, np.float32)
+    _check(_lib().g2h_model_load(path.encode(), C.byref(n), p, nparams), "g2h_model_load")
+    return Standardizer(norm=n), p
 
 
 def simulated_makespan(cost: np.ndarray, order: np.ndarray, split: np.ndarray, waves: int) -> float:

@@ -78,6 +78,9 @@ typedef struct g2_stats {
     uint64_t local_pass_roots[5]; /* local-bitmap matching: roots each pass took to the end (rows in
                                      a wave's LDS; in a 256-, 512-, 1024-thread workgroup's LDS;
                                      in per-wave global arenas) */
+    uint64_t host_spill_bytes;    /* MCE: page-locked host memory the large-root arenas used because
+                                     the device budget could not hold one wave's arena (0 = none) */
+    uint64_t device_bytes;        /* device memory the context holds after the run */
 } g2_stats;
 
 /* root schedule for g2_mce_run (NULL = the given source order, one wave per root) */

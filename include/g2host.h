@@ -127,6 +127,33 @@ void g2h_aimd_init(g2h_aimd* a);
 /* returns the reference's trace prefix: "--- chunk: ", "+++ chunk: " or "... chunk: " */
 const char* g2h_aimd_adjust(g2h_aimd* a, int overflow);
 
+/* ---- learned root scheduler, host side (no counterpart in the reference;
+ * the device side is g2_features / g2_mlp_* in g2hip.h) ---- */
+#define G2H_NFEAT_INT 5   /* integer device features per root (g2_features) */
+#define G2H_NFEAT     8   /* cost-model inputs per root */
+/* core number of every vertex (bucket peel, any CSR with sorted rows) */
+int  g2h_core_numbers(const uint64_t* row_ptrs, uint64_t nv, const uint32_t* cols, uint32_t* core);
+/* the cost model's 8 inputs of ns roots, float32 row-major: log1p(degree),
+ * log1p(core number), log1p(2-paths), log1p(triangles), local clustering
+ * 2t/(d(d-1)), id/nv (degeneracy rank of a kcore-recoded graph), shared-
+ * neighbour fraction, log1p(later neighbours); feat = g2_features' ns x 5 */
+int  g2h_root_features(const int64_t* feat, const uint32_t* sources, uint64_t ns, const uint32_t* core,
+                       uint64_t nv, float* out);
+/* per-feature standardisation fitted on training rows (population sd,
+ * floored at 1e-6) */
+typedef struct g2h_norm { float mu[G2H_NFEAT], sd[G2H_NFEAT]; } g2h_norm;
+int  g2h_norm_fit(const float* x, uint64_t n, g2h_norm* out);
+void g2h_norm_apply(const g2h_norm* norm, float* x, uint64_t n);  /* in place */
+/* processing order (longest predicted cost first, stable) and per-root split
+ * counts from predicted log(1 + subtree nodes): a root above the per-wave
+ * share of the predicted total is dealt into ceil(cost / share) parts, at
+ * most later[i] (its possible first-level children) and max_split (0 = 4096) */
+int  g2h_learned_schedule(const float* pred_log_nodes, const int64_t* later, uint64_t ns, uint64_t waves,
+                          uint32_t max_split, uint32_t* order, uint32_t* split);
+/* trained model file (standardiser + MLP parameters), text, exact round trip */
+int  g2h_model_save(const char* path, const g2h_norm* norm, const float* params, uint64_t nparams);
+int  g2h_model_load(const char* path, g2h_norm* norm, float* params, uint64_t nparams);
+
 const char* g2h_last_error(void);
 /* progress lines on stderr from long host steps (generators, CSR build,
  * preprocess, kcore); off by default */

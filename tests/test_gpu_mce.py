@@ -237,3 +237,32 @@ def test_schedule_checks_and_ranges(g2, orc, gpu_exec):
         nodes += r["per_root_nodes"]
     assert tot == ref["total"] and np.array_equal(hist, ref["hist"])
     assert np.array_equal(nodes, ref["per_root_nodes"])
+
+
+def test_host_spill_tiny_budget(g2, orc, cfg1):
+    """A budget that cannot hold one large-root arena: the large-root pass
+    runs with its arenas in page-locked host memory (app_BK/BK.h:357-398's
+    spill, rebuilt), and every count equals the oracle."""
+    print("case: host spill", flush=True)
+    g, _ = cfg1
+    tune = dict(wave_scratch_words=256, big_waves=64)  # most roots above degree ~20 are deferred
+    with g2.Executor(device=0, **tune) as ex:
+        ex.transfer(g)
+        free = ex.run_mce()
+    st = free["stats"]
+    need, nbig = st["max_scratch_words"], st["big_roots"]
+    assert nbig > 0 and st["host_spill_bytes"] == 0
+    bwaves = min(64, nbig)
+    bw = min(4, bwaves)
+    big = -(-(bwaves // bw * bw) * need * 8 // 256) * 256
+    budget = st["device_bytes"] - big + need * 4  # everything but half of one arena
+    with g2.Executor(device=0, mem_budget=budget, **tune) as ex:
+        ex.transfer(g)
+        got = ex.run_mce(per_root=True)
+    ref = orc.mce(g, per_root=True)
+    print(f"budget {budget} B, host spill {got['stats']['host_spill_bytes']} B, {nbig} large roots", flush=True)
+    assert got["stats"]["host_spill_bytes"] > 0
+    assert got["stats"]["device_bytes"] <= budget
+    assert got["total"] == ref["total"] == free["total"]
+    assert np.array_equal(got["hist"], ref["hist"])
+    assert np.array_equal(got["per_root_nodes"], ref["per_root_nodes"])

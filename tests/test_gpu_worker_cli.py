@@ -19,10 +19,10 @@ BIN = os.path.join(ROOT, "g2-aimd_amd")
 SEED = 0x6732A1D0C0000011
 
 
-def run_cli(name, args, out_json):
+def run_cli(name, args, out_json, timeout=90):
     cmd = [os.path.join(BIN, name)] + args + ["-json", str(out_json)]
     print("case:", " ".join(cmd), flush=True)
-    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=90)
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout)
     log = p.stdout.decode(errors="replace")
     assert p.returncode == 0, log[-3000:]
     with open(out_json) as f:
@@ -61,3 +61,54 @@ def test_run_matches_oracle(g2, orc, binned_graph, q):
     ref = orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
     assert r["app"] == "run" and r["query"] == q and r["total"] == ref > 0
     assert f"Total counts: {ref}" in log
+
+
+def fixture_entry(name):
+    with open(os.path.join(ROOT, "tests", "golden", "headline_counts.json")) as f:
+        return json.load(f)[name]
+
+
+def test_bk_learned_social_pipeline(g2, tmp_path):
+    """The social MCE workload through the drop-in pipeline: 4 kcore bins,
+    4 producers, 4 consumers, queue 4, sorted sources; the plain schedule and
+    the learned + batched one (cost model trained by bk itself on two other
+    seeds) both give the oracle's total and histogram (fixture)."""
+    fx = fixture_entry("social-1000000")
+    model = tmp_path / "social.model"
+    r, log = run_cli("bk", ["-gen", "social:1000000", "-h", "2", "-pn", "4", "-cn", "4", "-qs", "4", "-ss", "1",
+                            "-sched", "both", "-train", str(model),
+                            "-train-gen", "social:500000:0x6732A1D0C0000021,social:500000:0x6732A1D0C0000022",
+                            "-epochs", "10"], tmp_path / "social.json", timeout=400)
+    assert r["vertices"] == fx["nv"] and 2 * r["edges"] == fx["ne"]
+    assert r["total"] == fx["max_cliques"]
+    assert {int(k): v for k, v in r["hist"].items()} == {int(k): v for k, v in fx["hist"].items()}
+    for form in ("plain", "learned"):
+        assert r[form]["total"] == fx["max_cliques"]
+        assert r[form]["hist"] == r["hist"]
+    assert r["learned"]["split_roots"] >= 0 and r["plain"]["donations"] == 0
+    assert model.exists() and "learned vs plain" in log
+    print(f"plain {r['plain']['seconds']:.3f}s kernel {r['plain']['kernel_ms']:.1f} ms idle "
+          f"{r['plain']['wave_idle_frac']:.3f}; learned {r['learned']['seconds']:.3f}s kernel "
+          f"{r['learned']['kernel_ms']:.1f} ms idle {r['learned']['wave_idle_frac']:.3f}", flush=True)
+    # the model file is reusable
+    r2, _ = run_cli("bk", ["-gen", "social:200000:0x6732A1D0C0000023", "-h", "2", "-pn", "2", "-cn", "2",
+                           "-qs", "2", "-sched", "both", "-model", str(model)], tmp_path / "again.json", timeout=200)
+    assert r2["plain"]["total"] == r2["learned"]["total"] > 0
+
+
+def test_bk_aimd_retry(orc, g2, tmp_path):
+    """A budget that holds everything but the whole view's deferred-root list:
+    the first launch overflows, the AIMD chunk loop (system/subgraph_container.h:
+    178-211) retries smaller chunks, and the total stays the oracle's."""
+    args = ["-gen", "chunglu:100000", "-nb", "1", "-h", "2", "-pn", "1", "-cn", "1", "-qs", "1"]
+    r, _ = run_cli("bk", args, tmp_path / "free.json")
+    g = g2.gen_chung_lu(100000, 2.3, 2.0, 2000.0, 1000000, 0x6732A1D0C0000001).preprocess()
+    g, _, _, _ = g.kcore_recode()
+    ref = orc.mce(g, hist_len=128)
+    assert r["total"] == ref["total"] and r["plain"]["retries"] == 0
+    budget = r["plain"]["device_bytes"] - 8 * g.nv + 8 * 20000
+    r2, log = run_cli("bk", args + ["-budget", repr(budget / 1e9)], tmp_path / "tight.json")
+    assert r2["plain"]["retries"] >= 1 and "Iteration failed!!" in log and "--- chunk: " in log
+    assert r2["total"] == ref["total"]
+    assert {int(k): v for k, v in r2["hist"].items()} == \
+        {int(k): int(ref["hist"][k]) for k in np.nonzero(ref["hist"])[0]}
