@@ -5,18 +5,22 @@ Workload (N=1): 4-clique matching (preset q=24, the reference's `run -q 24`)
 over every root of a seeded R-MAT graph, scale 22, edge factor 16,
 (a,b,c,d) = (.57,.19,.19,.05), symmetrised, deduplicated, self loops and
 degree-0 vertices removed -- the R-MAT scale-22 matching workload, resident
-in HBM as one view.  One step = one full g2_gm_run over all roots.
-Also reported: triangles (q=0) on the same graph and maximal cliques on the
-Chung-Lu n=100k graph (secondary lines inside the JSON object).
+in HBM as one view.  One step = one full g2_gm_run over all roots (the
+oriented-row build and the counting passes).  The count is checked against
+the oracle's total for this graph (tests/golden/headline_counts.json).
+Also reported: triangles (q=0) on the same graph, maximal cliques on the
+Chung-Lu n=100k graph, and the learned-vs-plain root schedule on the R-MAT
+MCE family (cost model trained on 8 seeds, a fresh test seed).
 
 value      = 4-clique matches / second of the timed steps (inputs resident in HBM)
-roofline   = modelled algorithmic bytes of one launch / its HIP-event duration;
-             the bytes are the reference traversal's byte model
-             (oracle/oracle.cpp), computed by the canonical kernel's
-             accounting variant on a seeded root sample (untimed) and scaled
-             to all roots; per-root bytes add up, so the scaling is unbiased
+roofline   = the dominant counting pass: its algorithmic bytes (the oriented
+             rows it must read, computed exactly by the library's accounting
+             kernel, G2_RUN_LOCAL_BYTES) / its HIP-event duration, against
+             the HBM peak; traffic = its calibrated rocprof FETCH_SIZE +
+             WRITE_SIZE per dispatch (profiles/)
 cpu_baseline = the oracle (C++/OpenMP restatement of app_gmatch/gmatch.h) on a
-             seeded random sample of roots, timed on this host's cores
+             seeded random sample of roots, timed on this host's cores:
+             one warm-up, then the median of 3
 """
 from __future__ import annotations
 
@@ -99,35 +103,77 @@ def model_bytes(g2, ex, g, plan, total, frac, seed=11):
     return int(per_match * total), note
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) recorded in
-    profiles/<name> by the rocprofv3 passes listed there; None if absent."""
+def profiled_traffic(name, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) recorded as
+    `<key>: <bytes>` in profiles/<name> by the rocprofv3 passes listed there;
+    None if absent."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
 
+def headline_fixture(name):
+    path = os.path.join(ROOT, "tests", "golden", "headline_counts.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get(name)
+
+
+def fingerprint(g):
+    w = np.arange(1, g.ne + 1, dtype=np.uint64)
+    return {"nv": int(g.nv), "ne": int(g.ne),
+            "cols_checksum": int(np.sum(g.cols.astype(np.uint64) * w, dtype=np.uint64))}
+
+
+def host_cpu():
+    """(model name, threads the oracle may use)"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def timed_median(fn, reps=3):
+    """one warm-up call, then the median wall time of `reps` calls"""
+    fn()
+    ts, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return out, float(np.median(ts))
+
+
 def cpu_baseline(orc, g, plan, target_s, seed=7):
-    """oracle on a seeded random root sample sized to ~target_s seconds"""
-    threads = os.cpu_count() or 1
+    """oracle on a seeded random root sample sized to ~target_s seconds per
+    run; one warm-up, then the median of 3"""
+    model, threads = host_cpu()
     rng = np.random.default_rng(seed)
     perm = rng.permutation(g.nv).astype(np.uint32)
     frac = 0.002
     while True:
         n = max(1, int(frac * g.nv))
         t0 = time.perf_counter()
-        r = orc.gm(g, plan, sources=np.sort(perm[:n]), threads=threads)
+        orc.gm(g, plan, sources=np.sort(perm[:n]), threads=threads)
         dt = time.perf_counter() - t0
         if dt >= target_s * 0.5 or n == g.nv:
             break
         frac = min(1.0, frac * max(2.0, target_s / max(dt, 1e-3)))
+    src = np.sort(perm[:n])
+    r, dt = timed_median(lambda: orc.gm(g, plan, sources=src, threads=threads))
     return {"value": r["count"] / dt, "unit": "4-clique matches/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n} of {g.nv} roots (seeded uniform sample), {r['count']} matches in {dt:.2f}s"}
+            "kind": "port", "cpu": model,
+            "sample": f"{n} of {g.nv} roots (seeded uniform sample), {r['count']} matches in {dt:.2f}s "
+                      "(median of 3 after a warm-up)"}
 
 
 NEVER = 0xFFFFFFFF
@@ -389,12 +435,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=22, help="R-MAT scale of the workload")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=5.0,
+                    help="oracle time per baseline run (one warm-up, then the median of 3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--gm-chunk", type=int, default=0)
-    ap.add_argument("--model-sample", type=float, default=0.02,
-                    help="root fraction accounted for the byte model (1 = exact, 0 = skip)")
+    ap.add_argument("--model-sample", type=float, default=0.0,
+                    help="also model the reference traversal's bytes on this root fraction (0 = skip)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain schedule experiment")
+    ap.add_argument("--learned-scale", type=int, default=13)
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per launch of the dominant kernel from rocprof "
                          "(profiles/), reported as roofline.traffic")
@@ -444,6 +493,9 @@ def main():
     log(f"device: {info}")
 
     g = build_rmat(g2, args.scale)
+    fx = headline_fixture(f"rmat-{args.scale}")
+    if fx is not None and fingerprint(g) != {k: fx[k] for k in ("nv", "ne", "cols_checksum")}:
+        raise RuntimeError("the generated graph is not the one the oracle totals were taken on")
     ex = g2.Executor(device=0, gm_chunk=args.gm_chunk)
     t0 = time.perf_counter()
     ex.transfer(g)
@@ -451,13 +503,42 @@ def main():
     plan24 = g2.plan_for_preset(24)
     ex.set_plan(plan24)
 
-    cnt, ms_step, kernel_ms = time_gm(ex, args.steps, args.warmup)
-    alg_bytes, sample_note = model_bytes(g2, ex, g, plan24, cnt, args.model_sample)
-    value = cnt / (ms_step / 1e3)
-    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9 if alg_bytes else None
+    for _ in range(args.warmup):
+        ex.run_gm()
+    ms, kms, pass_ms, cnt = [], [], [], None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        r = ex.run_gm()
+        ms.append((time.perf_counter() - t0) * 1e3)
+        kms.append(r["stats"]["kernel_ms"])
+        pass_ms.append(r["stats"]["local_pass_ms"])
+        if cnt is not None and r["count"] != cnt:
+            raise RuntimeError("count changed between steps")
+        cnt = r["count"]
+        log(f"step {i}: {cnt} matches, counting passes {kms[-1]:.2f} ms, wall {ms[-1]:.2f} ms")
+    ms_step, kernel_ms = float(np.mean(ms)), float(np.mean(kms))
+    pinned = None
+    if fx is not None and "q24" in fx:
+        if cnt != fx["q24"]:
+            raise RuntimeError(f"4-clique count {cnt} differs from the oracle's {fx['q24']}")
+        pinned = "equal to the oracle's total (tests/golden/headline_counts.json)"
+    # algorithmic bytes of each counting pass (accounting kernel, untimed)
+    acc = ex.run_gm(local_bytes=True)
+    assert acc["count"] == cnt and acc["stats"]["local_bitmap"] == 1
+    pbytes = acc["stats"]["local_pass_bytes"]
+    pms = np.mean(np.array(pass_ms), axis=0)
+    names = ["gm_local_kernel<4> (rows in the wave's LDS)", "gm_local_block_kernel<4,256,...>",
+             "gm_local_block_kernel<4,512,...>", "gm_local_block_kernel<4,1024,...>",
+             "gm_local_kernel<4> (rows in global arenas)"]
+    passes = [{"kernel": names[k], "ms": float(pms[k]), "alg_bytes": int(pbytes[k]),
+               "achieved_GBs": pbytes[k] / (pms[k] / 1e3) / 1e9 if pms[k] > 0 else None,
+               "roots": int(acc["stats"]["local_pass_roots"][k])} for k in range(5)]
+    dom = int(np.argmax(pms))
+    achieved = passes[dom]["achieved_GBs"]
     traffic = args.traffic_bytes
     if traffic is None and args.scale == 22:
-        traffic = profiled_traffic("gm_local_kernel_q24_rmat22.md")
+        traffic = profiled_traffic("gm_local_kernel_q24_rmat22.md", f"traffic_bytes_per_launch_pass{dom + 1}")
+    value = cnt / (ms_step / 1e3)
     out = {
         "metric": METRIC,
         "value": value,
@@ -469,30 +550,39 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 vertex ids, u64 counts (integer set intersection)",
+        "dtype": "u32 vertex ids, u64 bitsets and counts (integer set intersection)",
         "data": "synthetic (seeded R-MAT, generated on the host)",
         "config": {
             "workload": f"4-clique matching (q=24) over all roots of R-MAT scale {args.scale}, "
                         f"edge factor 16, (.57,.19,.19,.05), one resident view",
-            "vertices": g.nv, "undirected_edges": g.ne // 2, "matches": cnt,
+            "vertices": g.nv, "undirected_edges": g.ne // 2, "matches": cnt, "matches_check": pinned,
             "gm_chunk": args.gm_chunk,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-            "traffic_GBs": traffic / (kernel_ms / 1e3) / 1e9 if traffic else None,
-            "traffic_frac": traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
-            "kernel": "gm_local_kernel<4> + gm_local_block_kernel<4, 256|1024, ...> (the counting passes)",
-            "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes,
-            "alg_bytes_source": sample_note,
+            "traffic_frac": traffic / (passes[dom]["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "kernel": passes[dom]["kernel"], "kernel_ms": passes[dom]["ms"],
+            "alg_bytes_per_launch": passes[dom]["alg_bytes"],
+            "alg_bytes_source": "oriented rows the pass must read: per root 16 B of row offsets, 4|U| B of U, "
+                                "and per member with a later member 16 B + 4 min(d, later x (1 + bitlen d)) B "
+                                "(G2_RUN_LOCAL_BYTES accounting kernel, equal to oracle/oracle.cpp orc_local_bytes)",
+            "passes": passes,
+            "counting_passes_ms": kernel_ms,
         },
         "upload_ms": upload_ms,
     }
+    if args.model_sample > 0:
+        ref_bytes, note = model_bytes(g2, ex, g, plan24, cnt, args.model_sample)
+        out["reference_traversal_bytes"] = {"bytes": ref_bytes, "note": note + "; the reference's "
+                                            "binary-search traversal, which the timed kernels do not perform"}
 
     if not args.no_extra:
         extra = {}
         ex.set_plan(g2.plan_for_preset(0))
         tri, tri_ms, tri_kms = time_gm(ex, 1, 1)
+        if fx is not None and tri != fx.get("q0", tri):
+            raise RuntimeError(f"triangle count {tri} differs from the oracle's {fx['q0']}")
         extra["triangles"] = {"count": tri, "ms": tri_ms, "per_s": tri / (tri_ms / 1e3)}
         ex.close()
         cl = g2.gen_chung_lu(100000, 2.3, 2.0, 2000.0, 1000000, CL_SEED).preprocess()
@@ -506,15 +596,37 @@ def main():
                 t0 = time.perf_counter()
                 r = mx.run_mce()
                 ts.append((time.perf_counter() - t0) * 1e3)
-            kms = r["stats"]["kernel_ms"]
+            kms_m = r["stats"]["kernel_ms"]
             extra["mce_chung_lu_100k"] = {
-                "max_cliques": r["total"], "ms": float(np.mean(ts)), "kernel_ms": kms,
+                "max_cliques": r["total"], "ms": float(np.mean(ts)), "kernel_ms": kms_m,
                 "per_s": r["total"] / (np.mean(ts) / 1e3),
                 "alg_bytes": acc_m["stats"]["alg_bytes"],
-                "achieved_GBs": acc_m["stats"]["alg_bytes"] / (kms / 1e3) / 1e9,
+                "achieved_GBs": acc_m["stats"]["alg_bytes"] / (kms_m / 1e3) / 1e9,
                 "note": "kernel_ms is the last of the 5 timed runs; the 9 MB CSR stays in "
                         "L2 and MALL, so achieved_GBs (modelled bytes) says nothing about HBM",
             }
+            if not args.no_cpu:
+                import oracle
+                model, threads = host_cpu()
+                ref, dt = timed_median(lambda: oracle.mce(cl, threads=threads))
+                if ref["total"] != r["total"] or not np.array_equal(ref["hist"], r["hist"]):
+                    raise RuntimeError("Chung-Lu 100k: the GPU result differs from the oracle's")
+                extra["mce_chung_lu_100k"]["cpu_baseline"] = {
+                    "value": ref["total"] / dt, "unit": "maximal cliques/s", "cores": threads, "kind": "port",
+                    "cpu": model, "sample": f"every root, {ref['total']} cliques in {dt:.2f}s (median of 3 "
+                                            "after a warm-up); equal to the GPU's total and histogram"}
+        if not args.no_learned:
+            S = importlib.import_module("g2-aimd_amd.sched")
+            ex5 = S.experiment(g2, dev, args.learned_scale, [0x6732A1D0C0002001 + i for i in range(8)],
+                               0x6732A1D0C0001002, epochs=40, batch=256, lr=3e-3, reps=3,
+                               log=lambda m: log("learned: " + m), mce_donate_every=64)
+            extra["learned_schedule_rmat"] = ex5
+            out["learned_vs_plain_speedup"] = ex5["speedup_learned_vs_plain"]
+            out["learned_vs_plain_note"] = (
+                f"maximal cliques on R-MAT scale {args.learned_scale} (cost model trained on 8 other seeds), "
+                "kernel time of the plain degeneracy order vs the learned order and splits, both with work "
+                "donation every 64 nodes; without donation: "
+                f"{ex5['speedup_learned_vs_plain_nodonate']:.3f}x")
         out["extra"] = extra
     else:
         ex.close()

@@ -157,18 +157,26 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
         if (cols[j] >= nv) return set_err(c, G2_EINVAL, "column id out of range");
     // Level-1 windows of the matching tasks: source i covers windows
     // [off[i], off[i+1]) of gm_chunk row entries each (one window when 0).
-    // Every neighbour of a source having its row in the view (a view of hop
-    // >= 2, or the whole graph) is what the local-bitmap matching path needs.
+    // The local-bitmap matching path needs every neighbour of a source that
+    // can root a match to have its row in the view: the largest degree of a
+    // source with a rowless neighbour is kept (a view materialised with the
+    // plan's hop and root degree leaves such neighbours only next to sources
+    // below the root degree, which root no match).
     std::vector<uint64_t> task_off(ns + 1, 0), src_deg(ns);
     const uint32_t chunk = c->tune.gm_chunk;
-    bool closed = true;
+    uint64_t unclosed = 0;
     for (uint64_t i = 0; i < ns; i++) {
         const uint64_t d = rp[src[i] + 1] - rp[src[i]];
         src_deg[i] = d;
         uint64_t w = 1;
         if (chunk && d > chunk) w = (d + chunk - 1) / chunk;
         task_off[i + 1] = task_off[i] + w;
-        for (uint64_t j = rp[src[i]]; closed && j < rp[src[i] + 1]; j++) closed = rp[cols[j] + 1] > rp[cols[j]];
+        if (d > unclosed)
+            for (uint64_t j = rp[src[i]]; j < rp[src[i] + 1]; j++)
+                if (rp[cols[j] + 1] == rp[cols[j]]) {
+                    unclosed = d;
+                    break;
+                }
     }
     // The budget is checked for every view buffer before any is replaced, so
     // a refused upload leaves the previous view intact.
@@ -216,7 +224,7 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     // whole-graph view: every vertex is a source, in id order
     c->sources_all = (ns == nv);
     for (uint64_t i = 0; c->sources_all && i < ns; i++) c->sources_all = (src[i] == i);
-    c->sources_closed = closed;
+    c->unclosed_max_degree = unclosed;
     c->gm_task_chunk = chunk;
     c->h_gm_task_off.swap(task_off);
     c->h_src_deg.swap(src_deg);

@@ -569,14 +569,16 @@ int gm_compile(g2_ctx* c, const g2_plan* p) {
 
 int gm_run(g2_ctx* c, uint32_t flags, uint64_t* count, uint64_t* per_root, g2_stats* st) {
     const bool account = flags & G2_RUN_ACCOUNT;
-    // totals of a root-dominated plan over the whole graph or a view whose
-    // sources' neighbours all have their rows: oriented local bitmaps
-    // (per-root counts and the byte model follow the reference's traversal)
+    // totals of a root-dominated plan over the whole graph or a view in
+    // which every source that can root a match (degree >= n-1) has the rows
+    // of all its neighbours: oriented local bitmaps (per-root counts and the
+    // byte model follow the reference's traversal)
     // (it keeps row offsets in 32 bits and sorts ranks as int-sized lists: fewer
     // than 2^32 row entries and 2^31 - 1 vertices)
-    if (c->local_ok && (c->sources_all || c->sources_closed) && c->ne < (1ull << 32) && c->nv + 1 <= 0x7fffffffull && !per_root &&
+    const bool closed = c->sources_all || c->unclosed_max_degree + 1 < (uint64_t)c->lprog.n;
+    if (c->local_ok && closed && c->ne < (1ull << 32) && c->nv + 1 <= 0x7fffffffull && !per_root &&
         !account && !(flags & G2_RUN_CANONICAL))
-        return gm_local_run(c, count, st);
+        return gm_local_run(c, count, st, flags & G2_RUN_LOCAL_BYTES);
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kGmMaxBlockWaves);
     uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM) / bw);
     const uint64_t cap = std::max<uint64_t>(c->max_degree, 64);

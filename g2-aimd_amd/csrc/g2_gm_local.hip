@@ -771,6 +771,56 @@ __global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__
     }
 }
 
+// Algorithmic bytes of the counting passes, per pass (the untimed
+// accounting run).  A root x reads its two row offsets (16 B); when it has
+// at least n-1 members it reads U = N+(x) (4m B) and, for every member U_i
+// with a later member, that member's row offsets (16 B) and the cheaper of
+// its whole oriented row and one binary search of it per later member:
+// 4 min(d_i, a_i (1 + bitlen(d_i))), d_i = |N+(U_i)|, a_i = m-1-i.  The
+// root's bytes go to the pass whose LDS rows it fits, by the rules the
+// passes apply (pass 1 takes every root of a 2-vertex plan).
+__device__ inline int local_pass_of(uint32_t n, uint64_t m) {
+    if (m < n - 1 || n == 2) return 0;
+    const uint64_t need = local_need(m);
+    if (m <= kLU && need <= kLRowWords) return 0;
+    if (m <= (uint64_t)kB1U && need <= (uint64_t)kB1RowWords) return 1;
+    if (m <= (uint64_t)kB2U && need <= (uint64_t)kB2RowWords) return 2;
+    if (m <= (uint64_t)kB3U && need <= (uint64_t)kB3RowWords) return 3;
+    return 4;
+}
+
+__global__ __launch_bounds__(256) void local_bytes_kernel(const uint64_t* __restrict__ orp,
+                                                          const uint32_t* __restrict__ ocols,
+                                                          const uint32_t* __restrict__ list, uint64_t nroots,
+                                                          uint32_t n, unsigned long long* __restrict__ out) {
+    const uint32_t lane = lane_id();
+    const uint64_t gwave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+    uint64_t acc[5] = {0, 0, 0, 0, 0};
+    for (uint64_t t = gwave; t < nroots; t += nw) {
+        const uint32_t x = list ? list[t] : (uint32_t)t;
+        const uint64_t lo = orp[x], m = orp[x + 1] - lo;
+        uint64_t b = 0;
+        if (n > 2 && m >= n - 1) {
+            for (uint64_t i = lane; i + 1 < m; i += kWave) {
+                const uint32_t u = ocols[lo + i];
+                const uint64_t d = orp[u + 1] - orp[u], later = m - 1 - i;
+                b += 16 + 4 * min<uint64_t>(d, later * (1 + bitlen(d)));
+            }
+            b = wave_sum(b);
+            b += 4 * m;
+        }
+        b += 16;
+        const int pass = local_pass_of(n, m);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if (k == pass) acc[k] += b;
+    }
+    if (lane == 0)
+        for (int k = 0; k < 5; k++)
+            if (acc[k]) atomicAdd(&out[k], (unsigned long long)acc[k]);
+}
+
 // Ranks of the sources: the roots of a view whose sources are a subset.
 __global__ __launch_bounds__(256) void oroot_kernel(const uint32_t* __restrict__ src, uint64_t ns,
                                                     const uint32_t* __restrict__ pi, uint32_t* __restrict__ roots) {
@@ -930,7 +980,7 @@ static void launch_local(LocalPass pass, uint64_t blocks, uint32_t bw, hipStream
 // 1024-thread workgroup's LDS; in per-wave global arenas sized to the largest
 // remaining root.  Each pass takes the roots the previous one deferred; the
 // deferred lists alternate between two buffers.
-int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
+int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st, bool bytes) {
     if (c->nv + 1 > 0x7fffffffull) return set_err(c, G2_EINVAL, "local-bitmap matching supports fewer than 2^31-1 vertices");
     const uint32_t bw = std::min<uint32_t>(c->tune.block_waves, kLocalBlockWaves);
     const uint64_t nblocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM_LOCAL) / bw);
@@ -979,6 +1029,8 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         return hipGetLastError();
     };
     unsigned long long h[LCTR_N];
+    double pass_ms[5] = {0, 0, 0, 0, 0};
+    int pass_no = 0;
     auto timed = [&](LocalPass pass, uint64_t blocks) -> int {
         if ((e = hipMemsetAsync(ctr + LCTR_NEXT, 0, 8, c->stream))) return hip_err(c, e, "memset next");
         if ((e = hipEventRecord(c->ev[1], c->stream))) return hip_err(c, e, "event");
@@ -989,6 +1041,7 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
         kernel_ms += ms;
+        pass_ms[pass_no] = ms;
         return G2_OK;
     };
     // pass 1: every root, rows in the wave's LDS
@@ -1007,6 +1060,7 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     const struct { LocalPass pass; int per_cu; } groups[3] = {
         {LocalPass::Group256, kB1PerCu}, {LocalPass::Group512, kB2PerCu}, {LocalPass::Group1024, kB3PerCu}};
     for (int g = 0; g < 3 && n; g++) {  // workgroup passes
+        pass_no = g + 1;
         a.list = lists[in];
         a.ntasks = n;
         a.big_list = lists[in ^ 1];
@@ -1019,6 +1073,7 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         in ^= 1;
     }
     if (n) {  // last pass: per-wave global arenas
+        pass_no = 4;
         uint64_t bwaves = std::min<uint64_t>(std::max<uint32_t>(c->tune.big_waves, 1), n);
         while (bwaves > 1 && c->budget && c->used - c->big_scratch.bytes + bwaves * need * 8 > c->budget) bwaves /= 2;
         bwaves = (bwaves + bw - 1) / bw * bw;
@@ -1036,6 +1091,17 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local sync");
     (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[3]);
     if (h[LCTR_ERR]) return set_err(c, G2_EBUDGET, "a root did not fit the large-root arena");
+    unsigned long long pass_bytes[5] = {0, 0, 0, 0, 0};
+    if (bytes) {  // accounting (untimed): the oriented rows each pass had to read
+        if ((e = hipMemsetAsync(ctr + LCTR_DEFER, 0, 5 * 8, c->stream))) return hip_err(c, e, "memset bytes");
+        const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((nroots + 3) / 4, (uint64_t)c->num_cus * 32));
+        hipLaunchKernelGGL(local_bytes_kernel, dim3(grid), dim3(256), 0, c->stream, orp, ocols,
+                           all ? nullptr : (const uint32_t*)c->roots.ptr, nroots, c->lprog.n, ctr + LCTR_DEFER);
+        if ((e = hipGetLastError())) return hip_err(c, e, "local bytes launch");
+        if ((e = hipMemcpyAsync(pass_bytes, ctr + LCTR_DEFER, sizeof(pass_bytes), hipMemcpyDeviceToHost, c->stream)))
+            return hip_err(c, e, "D2H bytes");
+        if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "local bytes sync");
+    }
     if (count) *count = h[LCTR_COUNT];
     if (st) {
         st->kernel_ms = kernel_ms;
@@ -1043,6 +1109,11 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st) {
         st->total_ms = tot;
         st->nodes = h[LCTR_NODES];
         st->alg_bytes = 0;
+        for (int k = 0; k < 5; k++) {
+            st->alg_bytes += pass_bytes[k];
+            st->local_pass_bytes[k] = pass_bytes[k];
+            st->local_pass_ms[k] = pass_ms[k];
+        }
         st->big_roots = n1;
         st->max_scratch_words = need;
         st->wave_idle_frac = 0;

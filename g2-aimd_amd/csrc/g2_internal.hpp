@@ -85,7 +85,7 @@ struct g2_ctx {
     uint64_t max_degree = 0;
     bool view_ok = false;         // the last upload completed (runs need a view)
     bool sources_all = false;     // sources == [0, nv) in order (whole-graph view)
-    bool sources_closed = false;  // every neighbour of a source has its row in the view
+    uint64_t unclosed_max_degree = 0;  // largest degree of a source with a rowless neighbour (0 = none)
     g2::DevBuf deg;               // u32 degree of every vertex
     std::vector<uint64_t> h_gm_task_off;  // host copy: level-1 window offsets per source
     g2::DevBuf gm_task_off;
@@ -149,7 +149,7 @@ int gm_program(const g2_plan* p, GmProgram* out, const char** why);
 // gm_kernel (no byte accounting) over `blocks` workgroups of block_waves waves
 hipError_t gm_launch(const GmArgs& a, const GmProgram& prog, uint64_t blocks, uint32_t block_waves, hipStream_t s);
 void gm_local_compile(g2_ctx* c, const g2_plan* p);
-int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st);
+int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st, bool bytes);
 int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
                  uint64_t* qnodes, uint32_t* over, g2_stats* st);
 

@@ -226,7 +226,12 @@ struct MlpArgs {
     float* grad;                        // G2_MLP_NPARAM accumulators (train)
     float* loss;                        // sum of squared errors (train)
     float* yout;                        // predictions (infer)
+    float* partial;                     // train: per wave, its gradient and squared-error sums
 };
+
+// per-wave partial sums: the G2_MLP_NPARAM gradient entries, then the sum
+// of squared errors, padded to a multiple of 4 floats
+constexpr uint32_t kPartialStride = (kNParam + 1 + 3) / 4 * 4;
 
 __device__ inline void load_tile(const MlpArgs& a, WaveTile& T, uint64_t base) {
     const uint32_t lane = lane_id();
@@ -345,7 +350,10 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpArgs a) {
         }
         wave_sync();
     }
-    // flush: reduce the column sums over the 4 row groups, then one atomic each
+    // flush: the wave's partial sums go to its own slot with plain stores
+    // (every entry is written by exactly one lane); mlp_reduce_kernel adds
+    // the slots in wave order, so training is reproducible bit for bit
+    float* P = a.partial + ((uint64_t)blockIdx.x * 4 + threadIdx.x / kWave) * kPartialStride;
     for (int t = 0; t < 4; t++) {
         float v3 = gw3[t], vb2 = gb2[t], vb1 = gb1[t];
         v3 += __shfl_xor(v3, 16, 64);
@@ -355,27 +363,39 @@ __global__ __launch_bounds__(256) void mlp_train_kernel(MlpArgs a) {
         vb1 += __shfl_xor(vb1, 16, 64);
         vb1 += __shfl_xor(vb1, 32, 64);
         if (q == 0) {
-            atomicAdd(&a.grad[oW3 + 16 * t + r], v3);
-            atomicAdd(&a.grad[oB2 + 16 * t + r], vb2);
-            atomicAdd(&a.grad[oB1 + 16 * t + r], vb1);
+            P[oW3 + 16 * t + r] = v3;
+            P[oB2 + 16 * t + r] = vb2;
+            P[oB1 + 16 * t + r] = vb1;
         }
     }
     for (int p = 0; p < 4; p++)
         for (int t = 0; t < 4; t++)
-            for (int i = 0; i < 4; i++) atomicAdd(&a.grad[oW2 + (16 * p + 4 * q + i) * kH + 16 * t + r], gw2[p][t][i]);
+            for (int i = 0; i < 4; i++) P[oW2 + (16 * p + 4 * q + i) * kH + 16 * t + r] = gw2[p][t][i];
     for (int t = 0; t < 4; t++)
         for (int i = 0; i < 4; i++) {
             const int k = 4 * q + i;
-            if (k < kIn) atomicAdd(&a.grad[oW1 + k * kH + 16 * t + r], gw1[t][i]);
+            if (k < kIn) P[oW1 + k * kH + 16 * t + r] = gw1[t][i];
         }
     for (int m = 32; m >= 1; m >>= 1) {  // lanes with r == 0 hold the partial sums
         gb3 += __shfl_xor(gb3, m, 64);
         sse += __shfl_xor(sse, m, 64);
     }
     if (lane == 0) {
-        atomicAdd(&a.grad[oB3], gb3);
-        atomicAdd(a.loss, sse);
+        P[oB3] = gb3;
+        P[kNParam] = sse;
     }
+}
+
+// grad[i] = sum over the launch's waves, in wave order, of their partial
+// entry i; the squared errors are added to *loss (an epoch's running sum)
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(const float* __restrict__ partial, uint32_t nwaves,
+                                                         float* __restrict__ grad, float* __restrict__ loss) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > (uint32_t)kNParam) return;
+    float s = 0;
+    for (uint32_t w = 0; w < nwaves; w++) s += partial[(uint64_t)w * kPartialStride + i];
+    if (i < (uint32_t)kNParam) grad[i] = s;
+    else *loss += s;
 }
 
 struct AdamArgs {
@@ -404,6 +424,8 @@ struct g2_mlp {
     hipStream_t stream = nullptr;
     float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
     float *x = nullptr, *y = nullptr, *out = nullptr;
+    float* partial = nullptr;  // per-wave partial gradients of one training launch
+    uint64_t partial_waves = 0;
     uint32_t* idx = nullptr;
     uint64_t cap = 0;
     uint64_t step = 0;
@@ -430,6 +452,28 @@ static int mlp_ensure(g2_mlp* M, uint64_t n) {
         return G2_EHIP;
     M->cap = n;
     return G2_OK;
+}
+
+// one training launch over n samples (x/idx as set in the arguments):
+// per-wave partial gradients, then their fixed-order reduction into grad
+static int mlp_train_launch(g2_mlp* M, MlpArgs a) {
+    const uint64_t blocks = std::min<uint64_t>((a.n + 63) / 64, (uint64_t)M->num_cus * 4);
+    const uint64_t waves = blocks * 4;
+    if (waves > M->partial_waves) {
+        if (M->partial) (void)hipFree(M->partial);
+        M->partial = nullptr;
+        M->partial_waves = 0;
+        if (hipMalloc(&M->partial, waves * kPartialStride * 4) != hipSuccess) {
+            M->partial = nullptr;
+            return G2_EHIP;
+        }
+        M->partial_waves = waves;
+    }
+    a.partial = M->partial;
+    hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
+    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, M->stream, M->partial,
+                       (uint32_t)waves, a.grad, a.loss);
+    return hipGetLastError() == hipSuccess ? G2_OK : G2_EHIP;
 }
 
 extern "C" {
@@ -481,7 +525,7 @@ void g2_mlp_destroy(g2_mlp* M) {
     if (!M) return;
     (void)hipSetDevice(M->device);
     for (void* p : {(void*)M->params, (void*)M->grad, (void*)M->m, (void*)M->v, (void*)M->loss, (void*)M->x,
-                    (void*)M->y, (void*)M->out, (void*)M->idx})
+                    (void*)M->y, (void*)M->out, (void*)M->idx, (void*)M->partial})
         if (p) (void)hipFree(p);
     if (M->stream) (void)hipStreamDestroy(M->stream);
     delete M;
@@ -503,7 +547,7 @@ int g2_mlp_infer(g2_mlp* M, const float* x, uint64_t n, float* y) {
     (void)hipSetDevice(M->device);
     if (mlp_ensure(M, n)) return G2_EHIP;
     if (hipMemcpyAsync(M->x, x, n * kIn * 4, hipMemcpyHostToDevice, M->stream)) return G2_EHIP;
-    MlpArgs a{M->params, M->x, nullptr, nullptr, n, 1.f, nullptr, nullptr, M->out};
+    MlpArgs a{M->params, M->x, nullptr, nullptr, n, 1.f, nullptr, nullptr, M->out, nullptr};
     const uint64_t blocks = std::min<uint64_t>((n + 63) / 64, (uint64_t)M->num_cus * 4);
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
     if (hipGetLastError() != hipSuccess) return G2_EHIP;
@@ -517,13 +561,10 @@ int g2_mlp_grad(g2_mlp* M, const float* x, const float* y, uint64_t n, float* gr
     (void)hipSetDevice(M->device);
     if (mlp_ensure(M, n)) return G2_EHIP;
     if (hipMemcpyAsync(M->x, x, n * kIn * 4, hipMemcpyHostToDevice, M->stream) ||
-        hipMemcpyAsync(M->y, y, n * 4, hipMemcpyHostToDevice, M->stream) ||
-        hipMemsetAsync(M->grad, 0, kNParam * 4, M->stream) || hipMemsetAsync(M->loss, 0, 4, M->stream))
+        hipMemcpyAsync(M->y, y, n * 4, hipMemcpyHostToDevice, M->stream) || hipMemsetAsync(M->loss, 0, 4, M->stream))
         return G2_EHIP;
-    MlpArgs a{M->params, M->x, M->y, nullptr, n, 1.f / n, M->grad, M->loss, nullptr};
-    const uint64_t blocks = std::min<uint64_t>((n + 63) / 64, (uint64_t)M->num_cus * 4);
-    hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
-    if (hipGetLastError() != hipSuccess) return G2_EHIP;
+    MlpArgs a{M->params, M->x, M->y, nullptr, n, 1.f / n, M->grad, M->loss, nullptr, nullptr};
+    if (mlp_train_launch(M, a)) return G2_EHIP;
     float sse = 0;
     if (hipMemcpyAsync(grad_out, M->grad, kNParam * 4, hipMemcpyDeviceToHost, M->stream) ||
         hipMemcpyAsync(&sse, M->loss, 4, hipMemcpyDeviceToHost, M->stream) || hipStreamSynchronize(M->stream))
@@ -551,10 +592,8 @@ int g2_mlp_train(g2_mlp* M, const float* x, const float* y, uint64_t n, uint32_t
             return G2_EHIP;
         for (uint64_t b0 = 0; b0 < n; b0 += batch) {
             const uint64_t bn = std::min<uint64_t>(batch, n - b0);
-            if (hipMemsetAsync(M->grad, 0, kNParam * 4, M->stream)) return G2_EHIP;
-            MlpArgs a{M->params, M->x, M->y, M->idx + b0, bn, 1.f / bn, M->grad, M->loss, nullptr};
-            const uint64_t blocks = std::min<uint64_t>((bn + 63) / 64, (uint64_t)M->num_cus * 4);
-            hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), 0, M->stream, a);
+            MlpArgs a{M->params, M->x, M->y, M->idx + b0, bn, 1.f / bn, M->grad, M->loss, nullptr, nullptr};
+            if (mlp_train_launch(M, a)) return G2_EHIP;
             M->step++;
             const float b1 = 0.9f, b2 = 0.999f;
             AdamArgs ad{M->params, M->grad, M->m, M->v, lr, b1, b2, 1e-8f,

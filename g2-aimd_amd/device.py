@@ -22,6 +22,7 @@ u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 G2_RUN_ACCOUNT = 1
 G2_RUN_CANONICAL = 2
+G2_RUN_LOCAL_BYTES = 4
 
 # every symbol include/g2hip.h declares
 EXPORTS = ("g2_create", "g2_set_tuning", "g2_destroy", "g2_last_error", "g2_upload_view", "g2_mce_run",
@@ -46,11 +47,13 @@ class Stats(C.Structure):
                 ("max_scratch_words", C.c_uint64), ("wave_idle_frac", C.c_double),
                 ("donations", C.c_uint64), ("local_bitmap", C.c_uint32), ("reserved", C.c_uint32),
                 ("local_pass_roots", C.c_uint64 * 5), ("host_spill_bytes", C.c_uint64),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("local_pass_ms", C.c_double * 5),
+                ("local_pass_bytes", C.c_uint64 * 5)]
 
     def to_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
-        d["local_pass_roots"] = list(self.local_pass_roots)
+        for k in ("local_pass_roots", "local_pass_ms", "local_pass_bytes"):
+            d[k] = list(getattr(self, k))
         return d
 
 
@@ -214,13 +217,16 @@ class Executor:
     def set_plan(self, plan) -> None:
         self._check(hip_lib().g2_gm_set_plan(self._ctx, C.byref(plan)), "g2_gm_set_plan")
 
-    def run_gm(self, per_root: bool = False, account: bool = False, canonical: bool = False) -> dict:
+    def run_gm(self, per_root: bool = False, account: bool = False, canonical: bool = False,
+               local_bytes: bool = False) -> dict:
         """Match count; whole-graph totals of clique-like plans use the oriented
-        local-bitmap kernel unless `canonical` (or per-root / byte accounting)."""
+        local-bitmap kernel unless `canonical` (or per-root / byte accounting).
+        `local_bytes`: the local path also reports its per-pass algorithmic bytes."""
         cnt = C.c_uint64()
         prc = np.zeros(max(self.ns, 1), np.uint64) if per_root else None
         st = Stats()
-        flags = (G2_RUN_ACCOUNT if account else 0) | (G2_RUN_CANONICAL if canonical else 0)
+        flags = ((G2_RUN_ACCOUNT if account else 0) | (G2_RUN_CANONICAL if canonical else 0) |
+                 (G2_RUN_LOCAL_BYTES if local_bytes else 0))
         rc = hip_lib().g2_gm_run(self._ctx, flags, C.byref(cnt),
                                  prc.ctypes.data if per_root else None, C.byref(st))
         self._check(rc, "g2_gm_run")

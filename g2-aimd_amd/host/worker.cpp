@@ -86,6 +86,7 @@ void RunResult::add(const RunResult& r) {
     nodes += r.nodes;
     donations += r.donations;
     split_roots += r.split_roots;
+    local_views += r.local_views;
     host_spill_bytes = std::max(host_spill_bytes, r.host_spill_bytes);
     device_bytes = std::max(device_bytes, r.device_bytes);
 }
@@ -226,6 +227,7 @@ RunResult Executor::run_gm() {
     r.launches = 1;
     r.nodes = st.nodes;
     r.device_bytes = st.device_bytes;
+    r.local_views = st.local_bitmap;
     return r;
 }
 
@@ -676,6 +678,11 @@ int run_worker(int argc, char** argv, App app) {
     for (auto& f : forms) {
         if (f.second.total.count != forms[0].second.total.count || f.second.total.hist != forms[0].second.total.hist)
             die("the " + f.first + " schedule changed the result");
+        // the two matching paths split a total over views differently
+        if (app == App::GM && f.second.total.local_views && f.second.total.local_views != f.second.total.launches)
+            die("views were counted by different matching paths (local bitmaps in " +
+                std::to_string(f.second.total.local_views) + " of " + std::to_string(f.second.total.launches) +
+                "); their per-view counts do not add up to the total");
     }
     const RunResult& res = forms[0].second.total;
     say("Total counts: " + std::to_string(res.count));

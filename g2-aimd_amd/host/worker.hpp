@@ -182,6 +182,7 @@ struct RunResult {
     uint32_t retries = 0;               // launches refused by the memory budget
     uint64_t nodes = 0, donations = 0, split_roots = 0;
     uint64_t host_spill_bytes = 0;      // largest host spill of a launch
+    uint32_t local_views = 0;           // GM: views counted by the local-bitmap path
     uint64_t device_bytes = 0;          // largest device memory held after a launch
     void add(const RunResult& r);
 };

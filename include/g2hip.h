@@ -81,6 +81,8 @@ typedef struct g2_stats {
     uint64_t host_spill_bytes;    /* MCE: page-locked host memory the large-root arenas used because
                                      the device budget could not hold one wave's arena (0 = none) */
     uint64_t device_bytes;        /* device memory the context holds after the run */
+    double   local_pass_ms[5];    /* local-bitmap matching: HIP-event time of each pass */
+    uint64_t local_pass_bytes[5]; /* with G2_RUN_LOCAL_BYTES: algorithmic bytes of each pass */
 } g2_stats;
 
 /* root schedule for g2_mce_run (NULL = the given source order, one wave per root) */
@@ -97,6 +99,9 @@ typedef struct g2_sched {
 #define G2_RUN_CANONICAL 2u       /* matching: always walk the reference's traversal order
                                      (otherwise whole-graph totals of clique-like plans use
                                      the degree-oriented local-bitmap kernel) */
+#define G2_RUN_LOCAL_BYTES 4u     /* matching on the local-bitmap path: also compute the bytes of
+                                     oriented rows each counting pass must read (alg_bytes,
+                                     local_pass_bytes; an extra untimed kernel) */
 
 int  g2_create(g2_ctx** ctx, int device, uint64_t mem_budget_bytes, const g2_tuning* tuning);
 /* Replace the context's launch tuning (0 fields = defaults) between runs, so
@@ -130,7 +135,15 @@ int  g2_mce_run(g2_ctx* ctx, const g2_sched* sched, uint32_t flags,
 int  g2_gm_set_plan(g2_ctx* ctx, const g2_plan* plan);
 
 /* Match count of the plan rooted at the uploaded sources (u64, the
- * overflow-free count of app_gmatch/gmatch.h).  per_root_count nullable. */
+ * overflow-free count of app_gmatch/gmatch.h).  per_root_count nullable.
+ * Totals of k-clique-like plans (g2_plan_local_eligible) take the oriented
+ * local-bitmap path when the view holds the rows of every neighbour of each
+ * source of degree >= n-1 (the whole graph, or a bin materialised with the
+ * plan's hop, view/view_bin.h:61-159): a view then counts the matches whose
+ * lowest (degree, id) vertex is one of its sources, where the reference's
+ * traversal counts those whose lowest-id vertex is; over bins that partition
+ * the vertices both give the whole-graph total.  stats->local_bitmap says
+ * which path ran: a total summed over views must not mix the two. */
 int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_count,
                g2_stats* stats);
 

@@ -91,6 +91,7 @@ def bind(path: str) -> C.CDLL:
     L.orc_kclique_count.argtypes = [u64p, C.c_uint64, u32p, C.c_uint32, C.c_int, P64]
     L.orc_kclique_count_ranked.argtypes = [u64p, C.c_uint64, u32p, C.c_uint32, u32p, C.c_uint64,
                                            C.c_int, P64]
+    L.orc_local_bytes.argtypes = [u64p, C.c_uint64, u32p, u32p, C.c_uint64, C.c_uint32, u64p]
     return L
 
 
@@ -175,3 +176,13 @@ def kclique_count_ranked(g, k: int, sources=None, threads: int = 0) -> int:
     rc = lib().orc_kclique_count_ranked(g.row_ptrs, g.nv, _cols(g), k, s, len(src), threads, C.byref(out))
     assert rc == 0, rc
     return out.value
+
+
+def local_bytes(g, n: int, sources=None) -> np.ndarray:
+    """per-pass algorithmic bytes of the local-bitmap matching kernels (u64[5])"""
+    src = _src(sources, g.nv)
+    s = src if len(src) else np.zeros(1, np.uint32)
+    out = np.zeros(5, np.uint64)
+    rc = lib().orc_local_bytes(g.row_ptrs, g.nv, _cols(g), s, len(src), n, out)
+    assert rc == 0, rc
+    return out

@@ -584,4 +584,62 @@ int orc_kclique_count_ranked(const uint64_t* rp, uint64_t nv, const uint32_t* co
     return 0;
 }
 
+// Algorithmic bytes of the local-bitmap matching passes (the byte model of
+// the kernels that count whole-graph totals of k-clique plans, not of the
+// reference's traversal): ranks by (degree, id), U(x) = later-ranked
+// neighbours of root x, m = |U(x)|, d_i = number of later-ranked neighbours
+// of the i-th member of U(x) in rank order, a_i = m-1-i.  A root costs 16 B
+// (its row offsets); with n > 2 and m >= n-1 also 4m (reading U) and, per
+// member with a later member, 16 + 4 min(d_i, a_i (1 + bitlen(d_i))): its
+// row offsets and the cheaper of its whole row or one binary search of it per
+// later member.  per_pass[k] splits the total by the pass whose LDS rows fit
+// the root (|U| <= 256 and m * stride <= 448 words; <= 448 and 3136;
+// <= 700 and 7700; <= 1040 and 17664; larger), stride = ceil(m/64) | 1,
+// with every root of a 2-vertex plan and every root with m < n-1 in the
+// first.
+int orc_local_bytes(const uint64_t* rp, uint64_t nv, const uint32_t* cols, const uint32_t* sources, uint64_t ns,
+                    uint32_t n, uint64_t* per_pass) {
+    if (n < 2 || n > 8) return -1;
+    for (uint64_t i = 0; i < ns; i++)
+        if (sources[i] >= nv) return -2;
+    auto deg = [&](uint32_t v) { return rp[v + 1] - rp[v]; };
+    auto later = [&](uint32_t a, uint32_t b) {  // key(b) > key(a)
+        return deg(b) > deg(a) || (deg(b) == deg(a) && b > a);
+    };
+    std::vector<uint64_t> up(nv, 0);  // later-ranked neighbours of each vertex
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t v = 0; v < (int64_t)nv; v++)
+        for (uint64_t e = rp[v]; e < rp[v + 1]; e++) up[v] += later((uint32_t)v, cols[e]);
+    uint64_t acc[5] = {0, 0, 0, 0, 0};
+    std::vector<uint32_t> U;
+    for (uint64_t t = 0; t < ns; t++) {
+        const uint32_t x = sources[t];
+        U.clear();
+        for (uint64_t e = rp[x]; e < rp[x + 1]; e++)
+            if (later(x, cols[e])) U.push_back(cols[e]);
+        std::sort(U.begin(), U.end(), [&](uint32_t a, uint32_t b) { return later(a, b); });
+        const uint64_t m = U.size();
+        uint64_t b = 16;
+        if (n > 2 && m >= n - 1) {
+            b += 4 * m;
+            for (uint64_t i = 0; i + 1 < m; i++) {
+                const uint64_t d = up[U[i]], a = m - 1 - i;
+                b += 16 + 4 * std::min(d, a * (1 + bitlen(d)));
+            }
+        }
+        int pass = 0;
+        if (n > 2 && m >= n - 1) {
+            const uint64_t need = m * (((m + 63) / 64) | 1);
+            if (m <= 256 && need <= 448) pass = 0;
+            else if (m <= 448 && need <= 3136) pass = 1;
+            else if (m <= 700 && need <= 7700) pass = 2;
+            else if (m <= 1040 && need <= 17664) pass = 3;
+            else pass = 4;
+        }
+        acc[pass] += b;
+    }
+    for (int k = 0; k < 5; k++) per_pass[k] = acc[k];
+    return 0;
+}
+
 }  // extern "C"

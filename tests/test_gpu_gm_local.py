@@ -112,10 +112,14 @@ def test_local_every_pass_takes_roots(g2, orc, gpu_exec):
         assert gpu_exec.run_gm(canonical=True)["count"] == ref
 
 
-@pytest.mark.parametrize("q,k", [(0, 3), (24, 4), (25, 5)])
-def test_local_view_bins(g2, orc, gpu_exec, q, k):
-    """Four bins of R-MAT 14 as hop-2 views, each on the local path."""
-    print(f"case: local path over view bins, q={q}", flush=True)
+@pytest.mark.parametrize("q,k,root_degree", [(0, 3, False), (24, 4, False), (25, 5, False), (24, 4, True),
+                                             (25, 5, True)])
+def test_local_view_bins(g2, orc, gpu_exec, q, k, root_degree):
+    """Four bins of R-MAT 14 as hop-2 views, each on the local path; with
+    root_degree, sources below the plan's root degree are not extended (as
+    the run command line materialises them), so some of them sit next to
+    rowless vertices."""
+    print(f"case: local path over view bins, q={q}, root degree {root_degree}", flush=True)
     g = g2.gen_rmat(14, 16, 0.57, 0.19, 0.19, RMAT_SEED).preprocess()
     plan = g2.plan_for_preset(q)
     whole = orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(plan)))["count"]
@@ -123,7 +127,7 @@ def test_local_view_bins(g2, orc, gpu_exec, q, k):
     total = 0
     for b in range(4):
         srcs = np.nonzero(bins == b)[0].astype(np.uint32)
-        view = g2.view_materialize(g, srcs, plan.hop)
+        view = g2.view_materialize(g, srcs, plan.hop, plan.root_degree if root_degree else 0)
         gpu_exec.transfer(view, srcs)
         gpu_exec.set_plan(plan)
         r = gpu_exec.run_gm()
@@ -165,3 +169,19 @@ def test_upload_over_budget_keeps_state_consistent(g2):
         with pytest.raises(RuntimeError, match=r"\(-3\)"):
             ex.transfer(big)
         assert ex.run_gm()["count"] == 495
+
+
+@pytest.mark.parametrize("q,n", [(0, 3), (24, 4), (25, 5)])
+def test_local_byte_model_equals_oracle(g2, orc, gpu_exec, q, n):
+    """The accounting kernel's per-pass algorithmic bytes equal the oracle's
+    restatement of the byte model, on R-MAT 12 and on the graph that fills
+    every pass."""
+    print(f"case: local byte model q={q}", flush=True)
+    for g in (g2.gen_rmat(12, 16, 0.57, 0.19, 0.19, RMAT_SEED).preprocess(), gadget_graph(g2, 5)):
+        gpu_exec.transfer(g)
+        gpu_exec.set_plan(g2.plan_for_preset(q))
+        r = gpu_exec.run_gm(local_bytes=True)
+        ref = orc.local_bytes(g, n)
+        assert r["stats"]["local_bitmap"] == 1
+        assert r["stats"]["local_pass_bytes"] == ref.tolist()
+        assert r["stats"]["alg_bytes"] == int(ref.sum())

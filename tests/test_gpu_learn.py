@@ -119,3 +119,78 @@ def test_learned_schedule_gain(g2, dev):
     learned_span = S.simulated_makespan(nodes, order, None, waves)
     print("makespan plain", plain_span, "learned order", learned_span, "bound", nodes.sum() / waves, flush=True)
     assert learned_span < 0.8 * plain_span
+    # measured: the same kernel and launch shape (one root per wave, no work
+    # donation), plain order against the learned order and splits
+    with dev.Executor(device=0, mce_donate_every=0xFFFFFFFF, mce_roots_per_wave=1) as ex:
+        ex.transfer(g)
+        t = {}
+        for name, kw in (("plain", {}), ("learned", {"order": order, "split": split})):
+            ex.run_mce(**kw)
+            t[name] = float(np.median([ex.run_mce(**kw)["stats"]["kernel_ms"] for _ in range(3)]))
+    print(f"measured kernel time: plain {t['plain']:.1f} ms, learned {t['learned']:.1f} ms "
+          f"({t['plain'] / t['learned']:.2f}x)", flush=True)
+    assert t["learned"] < 0.9 * t["plain"]
+
+
+def test_learned_schedule_rmat_family(g2, orc, dev):
+    """Learned root scheduling on the R-MAT family: the cost model trained on
+    plain-run logs of 8 seeds (scale 12) schedules a fresh seed.  Total,
+    histogram and every root's subtree node count equal the oracle's under
+    the learned order and splits, with and without work donation; the
+    prediction error, rank correlation, wave idle fractions and measured
+    kernel times of both schedules are reported."""
+    print("case: learned schedule, rmat family", flush=True)
+    S = importlib.import_module("g2-aimd_amd.sched")
+    xs, ys = [], []
+    for i in range(8):
+        g, core = S.prepare_graph(g2, 12, 0x6732A1D0C0002001 + i)
+        x, y, _ = S.logged_rows(dev, g, core)
+        xs.append(x)
+        ys.append(y)
+    X, Y = np.concatenate(xs), np.concatenate(ys)
+    norm = S.Standardizer(X)
+    m = dev.Mlp(seed=3)
+    m.train(norm(X), Y, epochs=40, batch=256, lr=3e-3)
+    g, core = S.prepare_graph(g2, 12, 0x6732A1D0C0001002)
+    ref = orc.mce(g, per_root=True)
+    truth = np.log1p(ref["per_root_nodes"].astype(np.float64))
+    waves = dev.device_info(0)["num_cus"] * 16
+    out = {}
+    for dname, tune in (("donate", {"mce_donate_every": 64}), ("nodonate", {"mce_donate_every": 0xFFFFFFFF})):
+        with dev.Executor(device=0, **tune) as ex:
+            ex.transfer(g)
+            x = S.root_features(ex, np.arange(g.nv, dtype=np.uint32), core, g.nv)
+            pred = m.infer(norm(x))
+            order, split = S.learned_schedule(pred, ex.features()[:, 4], waves)
+            for sname, kw in (("plain", {}), ("learned", {"order": order, "split": split})):
+                r = ex.run_mce(per_root=True, **kw)
+                assert r["total"] == ref["total"]
+                assert np.array_equal(r["hist"], ref["hist"])
+                assert np.array_equal(r["per_root_nodes"], ref["per_root_nodes"])
+                st = [ex.run_mce(**kw)["stats"] for _ in range(3)]
+                out[f"{sname}_{dname}"] = (float(np.median([s["kernel_ms"] for s in st])),
+                                           float(np.median([s["wave_idle_frac"] for s in st])))
+    m.close()
+    rmse = float(np.sqrt(np.mean((pred - truth) ** 2)))
+    rho = S.spearman(pred, truth)
+    print(f"rmse {rmse:.3f} spearman {rho:.3f} " +
+          " ".join(f"{k}: {v[0]:.1f} ms idle {v[1]:.3f}" for k, v in out.items()), flush=True)
+    assert rho > 0.8
+
+
+def test_mlp_training_is_reproducible(dev):
+    """Per-wave partial gradients reduced in a fixed order: the same seed and
+    data give the same parameters bit for bit, and so does the gradient."""
+    print("case: mlp reproducible", flush=True)
+    rng = np.random.default_rng(12)
+    x = rng.normal(0, 1, (30000, 8)).astype(np.float32)
+    t = (np.tanh(x[:, 0]) + 0.3 * x[:, 1] * x[:, 2]).astype(np.float32)
+    params, grads = [], []
+    for _ in range(2):
+        m = dev.Mlp(seed=21)
+        grads.append(m.grad(x, t))
+        m.train(x, t, epochs=3, batch=1024, lr=2e-3)
+        params.append(m.params())
+        m.close()
+    assert np.array_equal(params[0], params[1])
+    assert grads[0][0] == grads[1][0] and np.array_equal(grads[0][1], grads[1][1])

@@ -122,3 +122,25 @@ def test_ranked_kclique_split(g2, orc):
     # relabelled so that the lowest id is the degree-3 vertex: the rule follows degree first
     tri2 = G.from_pairs(g2, [(1, 2), (1, 0), (2, 0), (0, 3)], 4)
     assert [orc.kclique_count_ranked(tri2, 3, sources=np.array([v], np.uint32)) for v in range(4)] == [0, 1, 0, 0]
+
+
+def test_local_bytes_hand_worked(g2, orc):
+    """The local-path byte model on K_4 with the 4-clique plan: all degrees
+    are 3, so ranks follow ids.  Root 0: U = {1,2,3}, m = 3 >= 3: 16 + 12 +
+    member 1 (d = 2, a = 2: 16 + 4 min(2, 2*3)) + member 2 (d = 1, a = 1:
+    16 + 4 min(1, 1*2)) = 72; roots 1, 2, 3 have m < 3: 16 each.  Total 120,
+    all in the first pass.  Triangles (n = 3): root 1 has U = {2,3}, m = 2:
+    16 + 8 + (16 + 4) = 44; root 0: 16 + 12 + 24 + 20 = 72; roots 2, 3: 16."""
+    k4 = G.complete(g2, 4)
+    assert orc.local_bytes(k4, 4).tolist() == [120, 0, 0, 0, 0]
+    assert orc.local_bytes(k4, 3).tolist() == [72 + 44 + 32, 0, 0, 0, 0]
+    assert orc.local_bytes(k4, 2).tolist() == [64, 0, 0, 0, 0]
+    # a star: the hub has the highest degree, every leaf's U is {hub}
+    star = G.from_pairs(g2, [(0, i) for i in range(1, 6)], 6)
+    assert orc.local_bytes(star, 3).tolist() == [6 * 16, 0, 0, 0, 0]
+    # K_{300,320}: each of the 320 degree-300 vertices ranks below the 300
+    # degree-320 ones, so its U is that whole side (m = 300, rows of stride 5:
+    # 1500 words, the second pass): 16 + 4*300 + 299 members with no later
+    # neighbour (16 each) = 6000 B; the 300 others have m = 0: 16 B each
+    big = G.complete_bipartite(g2, 300, 320)
+    assert orc.local_bytes(big, 3).tolist() == [300 * 16, 320 * 6000, 0, 0, 0]
