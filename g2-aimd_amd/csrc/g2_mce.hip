@@ -1409,12 +1409,14 @@ int mce_run(g2_ctx* c, const g2_sched* sched, uint32_t flags, uint64_t* total, u
         first = sched->begin;
         ntasks = sched->end;
     }
-    if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks - first, 1) * 8, "deferred roots"))) return rc;
     // donation queue: counters, record table and a word pool (a record is a frame plus two words)
     const uint32_t every = account ? 0u : (c->tune.mce_donate_every == 0xffffffffu ? 0u : c->tune.mce_donate_every);
     const uint64_t max_records = 1ull << 16, pool_words = 1ull << 23;
     const uint64_t dq_words = DQ_N * kDqStride;
     if ((rc = ensure(c, c->donate, dq_words * 8 + max_records * 12 + pool_words * 8, "donation queue"))) return rc;
+    // the one buffer that grows with the number of root tasks comes last, so
+    // that a smaller range of roots fits where the whole view did not
+    if ((rc = ensure(c, c->list, std::max<uint64_t>(ntasks - first, 1) * 8, "deferred roots"))) return rc;
     auto* dq = (unsigned long long*)c->donate.ptr;
     auto* rec_off = dq + dq_words;
     auto* rec_ready = (unsigned int*)(rec_off + max_records);

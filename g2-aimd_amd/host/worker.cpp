@@ -187,7 +187,8 @@ RunResult Executor::run_bk(uint32_t hist_len, const BkForm& form) {
         r.retries++;
         g2h_aimd aimd;
         g2h_aimd_init(&aimd);
-        say(std::string(g2h_aimd_adjust(&aimd, 1)) + std::to_string(aimd.chunk));
+        const char* tag0 = g2h_aimd_adjust(&aimd, 1);
+        say(std::string(tag0) + std::to_string(aimd.chunk));
         for (uint64_t i = 0; i < ns_;) {
             g2_sched s = sched;
             s.begin = i;
@@ -531,6 +532,7 @@ int run_worker(int argc, char** argv, App app) {
     const int dst = cmd.get_int("-dst", 1);
     const int ss = cmd.get_int("-ss", 0);
     const int nb = cmd.get_int("-nb", 4);
+    const std::string bin_kind = cmd.get("-bins", "kcore");
     const double budget_gb = cmd.get_double("-budget", 0);  // per consumer, 0 = share the device evenly
     const std::string json = cmd.get("-json", "");
     const std::string sched = cmd.get("-sched", "plain");
@@ -554,6 +556,7 @@ int run_worker(int argc, char** argv, App app) {
     if (want_learned && model_in.empty() && train_out.empty()) die("-sched " + sched + " needs -model or -train");
     if (!train_out.empty() && train_gen.empty()) die("-train needs -train-gen <spec>[,<spec>...]");
     if (nb < 1) die("-nb must be >= 1");
+    if (bin_kind != "kcore" && bin_kind != "kmeans") die("-bins must be kcore or kmeans");
     {
         std::ostringstream o;
         o << "m: " << mem << " t: " << threads << " qs: " << qs << " pn: " << pn << " cn: " << cn;
@@ -609,8 +612,18 @@ int run_worker(int argc, char** argv, App app) {
     std::vector<ViewBin> pool;
     if (!gen.empty()) {
         std::vector<uint32_t> bins(std::max<uint64_t>(nv, 1));
-        if (g2h_kcore_bins(shells.data(), shells.size() - 1, (uint32_t)nb, bins.data(), nv) != 0)
+        if (bin_kind == "kmeans") {
+            // k-means view binning (kmeans/kmeans.cpp), one view per cluster
+            Timer kt;
+            kt.start();
+            uint32_t ncl = 0;
+            if (g2h_kmeans(rp, nv, g2h_graph_cols(g), (uint32_t)nb, 1, 20, 0x6732A1D0C000000Bull, bins.data(), &ncl) != 0)
+                die("kmeans: " + std::string(g2h_last_error()));
+            kt.stop();
+            say(kt.line("kmeans view binning, " + std::to_string(ncl) + " clusters"));
+        } else if (g2h_kcore_bins(shells.data(), shells.size() - 1, (uint32_t)nb, bins.data(), nv) != 0) {
             die("kcore bins: " + std::string(g2h_last_error()));
+        }
         pool.resize(nb);
         for (int b = 0; b < nb; b++) pool[b].id = b;
         for (uint64_t v = 0; v < nv; v++) pool[bins[v]].sources.push_back((uint32_t)v);

@@ -22,7 +22,9 @@
 // Additions (no reference counterpart):
 //   -gen social:<n>[:seed] | rmat:<scale>[:seed] | chunglu:<n>[:seed]
 //        a seeded synthetic graph instead of -dg, degeneracy-recoded and cut
-//        into -nb (default 4) kcore bins in memory (no files)
+//        into -nb (default 4) bins in memory (no files): -bins kcore (the
+//        kcore tool's blocks) or -bins kmeans (k-means view binning, 20
+//        rounds, fixed seed)
 //   -sched plain | learned | both     (bk) root schedule of every view:
 //        plain = source order, one root per wave, no work donation;
 //        learned = cost-model order and heavy-root splits, work donation,

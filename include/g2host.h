@@ -61,6 +61,30 @@ int  g2h_kcore_bins(const uint64_t* shell_offsets, uint64_t num_shells, uint32_t
 int  g2h_write_vbmap(const char* path, uint32_t nbins, const uint32_t* bins, uint64_t nv);
 int  g2h_read_vbmap(const char* path, uint32_t* nbins, uint32_t* bins, uint64_t nv);
 
+/* ---- offline input tools (the reference kmeans directory) ---- */
+/* SNAP text edge list (common/graph.h:268-358): lines starting with a digit
+ * hold "a b"; ids shifted by the smallest; both directions stored, rows
+ * sorted; parallel edges and self loops kept (g2h_graph_preprocess drops them) */
+int  g2h_graph_read_snap(const char* path, g2h_graph** out);
+/* k-means view binning (kmeans/kmeans.cpp:53-795): cosine similarity of
+ * adjacency rows scaled by 1/sqrt(degree), dense centroids, seeded
+ * farthest-first initialisation, `iters` rounds; k2 > 1 clusters each of
+ * the k1 clusters again (the reference's "k1xk2").  cluster[v] in
+ * [0, *nclusters), empty clusters removed. */
+int  g2h_kmeans(const uint64_t* row_ptrs, uint64_t nv, const uint32_t* cols, uint32_t k1, uint32_t k2,
+                uint32_t iters, uint64_t seed, uint32_t* cluster, uint32_t* nclusters);
+/* view packing (kmeans/view_pack.cpp:123-328, kmeans/view_bin.h): each
+ * cluster's vertices (id order, or by decreasing degree) go to the open bin
+ * whose hop closure grows least within max_entries row entries, pool_size
+ * bins open at a time; bins then merged next-fit by size.  bins[v] in
+ * [0, *nbins). */
+int  g2h_view_pack(const uint64_t* row_ptrs, uint64_t nv, const uint32_t* cols, const uint32_t* cluster,
+                   uint32_t nclusters, int hop, uint64_t max_entries, uint32_t pool_size, int by_degree,
+                   uint32_t* bins, uint32_t* nbins);
+/* .vbmap with one id per line, as view_pack writes it (kmeans/view_pack.cpp:316-322);
+ * g2h_read_vbmap reads both layouts */
+int  g2h_write_vbmap_lines(const char* path, uint32_t nbins, const uint32_t* bins, uint64_t nv);
+
 /* ---- synthetic workloads (seeded: splitmix64 -> xoshiro256**) ---- */
 /* Chung-Lu: w_i = w_max*(1+i/i0)^(-1/(gamma-1)) with i0 solved so that
  * sum(w) = 2*target_edges, clipped below at w_min; round(sum/2) endpoint
