@@ -9,6 +9,8 @@
 //   view_pack -gf G.bin -pf P -h 2 -m GB -s pool -o 0|1 -d 1
 //                                      -> <G>.bin.<h>.hop.vbmap and .vbstat
 //                                         (kmeans/view_pack.cpp:330-365)
+//   toGAMMA G | toGSI G | toPangolin G -> the graph in those systems' input
+//                                         formats (converter/*.cpp)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -143,6 +145,60 @@ int view_pack(int argc, char** argv) {
     return 0;
 }
 
+// converter/toGAMMA.cpp: <prefix>.col (u32 V, u64 row_ptrs[V+1]),
+// <prefix>.dst (u64 E, u32 cols[E]), <prefix>.vlabel (u32 V, V bytes of label 1)
+int to_gamma(const std::string& f) {
+    g2h_graph* g = load(f);
+    const std::string pre = prefix_of(f);
+    const uint64_t nv = g2h_graph_nv(g), ne = g2h_graph_ne(g);
+    const uint32_t v32 = (uint32_t)nv;
+    FILE* o = std::fopen((pre + ".col").c_str(), "wb");
+    if (!o) fail("cannot write " + pre + ".col");
+    std::fwrite(&v32, 4, 1, o);
+    std::fwrite(g2h_graph_row_ptrs(g), 8, nv + 1, o);
+    std::fclose(o);
+    o = std::fopen((pre + ".dst").c_str(), "wb");
+    if (!o) fail("cannot write " + pre + ".dst");
+    std::fwrite(&ne, 8, 1, o);
+    if (ne) std::fwrite(g2h_graph_cols(g), 4, ne, o);
+    std::fclose(o);
+    o = std::fopen((pre + ".vlabel").c_str(), "wb");
+    if (!o) fail("cannot write " + pre + ".vlabel");
+    std::fwrite(&v32, 4, 1, o);
+    const std::vector<uint8_t> label(nv, 1);
+    if (nv) std::fwrite(label.data(), 1, nv, o);
+    std::fclose(o);
+    g2h_graph_free(g);
+    return 0;
+}
+
+// converter/toGSI.cpp: <prefix>.g, "t # 0", "V E 1 1", a "v i 1" line per
+// vertex, an "e i j 1" line per edge (i < j), "t # -1"
+// converter/toPangolin.cpp: <prefix>.pangolin.txt, "i+1 j+1" per edge (i < j)
+int to_text(const std::string& f, bool gsi) {
+    g2h_graph* g = load(f);
+    const std::string out = prefix_of(f) + (gsi ? ".g" : ".pangolin.txt");
+    const uint64_t nv = g2h_graph_nv(g), ne = g2h_graph_ne(g);
+    const uint64_t* rp = g2h_graph_row_ptrs(g);
+    const uint32_t* cols = g2h_graph_cols(g);
+    FILE* o = std::fopen(out.c_str(), "w");
+    if (!o) fail("cannot write " + out);
+    if (gsi) {
+        std::fprintf(o, "t # 0\n%llu %llu 1 1\n", (unsigned long long)nv, (unsigned long long)(ne / 2));
+        for (uint64_t i = 0; i < nv; i++) std::fprintf(o, "v %llu 1\n", (unsigned long long)i);
+    }
+    for (uint64_t i = 0; i < nv; i++)
+        for (uint64_t j = rp[i]; j < rp[i + 1]; j++)
+            if (i < cols[j]) {
+                if (gsi) std::fprintf(o, "e %llu %u 1\n", (unsigned long long)i, cols[j]);
+                else std::fprintf(o, "%llu %llu\n", (unsigned long long)i + 1, (unsigned long long)cols[j] + 1);
+            }
+    if (gsi) std::fprintf(o, "t # -1\n");
+    std::fclose(o);
+    g2h_graph_free(g);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -152,6 +208,13 @@ int main(int argc, char** argv) {
     if (name == "kcore") return kcore(argc, argv);
     if (name == "kmeans") return kmeans(argc, argv);
     if (name == "view_pack") return view_pack(argc, argv);
-    std::printf("call this program as preprocess, kcore, kmeans or view_pack\n");
+    if (name == "toGAMMA" || name == "toGSI" || name == "toPangolin") {
+        if (argc < 2) {
+            std::printf("usage: %s <graph file>\n", name.c_str());
+            return -1;
+        }
+        return name == "toGAMMA" ? to_gamma(argv[1]) : to_text(argv[1], name == "toGSI");
+    }
+    std::printf("call this program as preprocess, kcore, kmeans, view_pack, toGAMMA, toGSI or toPangolin\n");
     return -1;
 }

@@ -506,6 +506,37 @@ void json_form(std::ostream& f, const PipelineResult& r) {
     f << "}";
 }
 
+// -qf: a pattern as text, one edge "a b" per line (ids from 0, at most 8
+// vertices, connected; lines not starting with a digit are skipped)
+void read_pattern(const std::string& path, uint32_t* n, uint32_t qadj[G2_MAXQ]) {
+    std::ifstream in(path);
+    if (!in) die("cannot read pattern file " + path);
+    std::string line;
+    uint32_t nv = 0;
+    for (int k = 0; k < G2_MAXQ; k++) qadj[k] = 0;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] < '0' || line[0] > '9') continue;
+        std::istringstream ss(line);
+        long a = -1, b = -1;
+        if (!(ss >> a >> b) || a < 0 || b < 0 || a >= G2_MAXQ || b >= G2_MAXQ || a == b)
+            die("pattern file " + path + ": bad edge line '" + line + "' (ids 0.." + std::to_string(G2_MAXQ - 1) + ")");
+        qadj[a] |= 1u << b;
+        qadj[b] |= 1u << a;
+        nv = std::max<uint32_t>(nv, (uint32_t)std::max(a, b) + 1);
+    }
+    if (nv < 2) die("pattern file " + path + " has no edge");
+    uint32_t seen = 1, frontier = 1;  // connected from vertex 0
+    while (frontier) {
+        uint32_t next = 0;
+        for (uint32_t v = 0; v < nv; v++)
+            if (frontier >> v & 1) next |= qadj[v];
+        frontier = next & ~seen;
+        seen |= next;
+    }
+    if (seen != (1u << nv) - 1) die("pattern file " + path + ": the pattern is not connected");
+    *n = nv;
+}
+
 std::vector<std::string> split_list(const std::string& s) {
     std::vector<std::string> out;
     std::stringstream ss(s);
@@ -535,6 +566,7 @@ int run_worker(int argc, char** argv, App app) {
     const std::string bin_kind = cmd.get("-bins", "kcore");
     const double budget_gb = cmd.get_double("-budget", 0);  // per consumer, 0 = share the device evenly
     const std::string json = cmd.get("-json", "");
+    const std::string query_file = cmd.get("-qf", "");
     const std::string sched = cmd.get("-sched", "plain");
     const std::string model_in = cmd.get("-model", "");
     const std::string train_out = cmd.get("-train", "");
@@ -598,10 +630,16 @@ int run_worker(int argc, char** argv, App app) {
     g2_plan plan{};
     uint64_t root_degree = 0;
     if (app == App::GM) {
-        if (query == 1000) die("run needs -q <pattern>");
+        if (query == 1000 && query_file.empty()) die("run needs -q <pattern> or -qf <pattern file>");
         if (hop != -1) die("-h is derived from the query plan for run");
         uint32_t n = 0, qadj[G2_MAXQ];
-        if (g2h_query_preset(query, &n, qadj) != 0) die(std::string("query: ") + g2h_last_error());
+        if (!query_file.empty()) {
+            uint32_t raw[G2_MAXQ] = {0};
+            read_pattern(query_file, &n, raw);
+            if (g2h_query_remap(n, raw, qadj) != 0) die(std::string("pattern: ") + g2h_last_error());
+        } else if (g2h_query_preset(query, &n, qadj) != 0) {
+            die(std::string("query: ") + g2h_last_error());
+        }
         if (g2h_plan_build(n, qadj, &plan) != 0) die(std::string("plan: ") + g2h_last_error());
         print_plan(plan, n, qadj);
         hop = plan.hop;

@@ -34,6 +34,9 @@
 //   -train <file>   train the cost model first, on plain runs of the
 //        -train-gen graphs (comma-separated specs), and write it to <file>
 //   -epochs <n>     training epochs (20)
+//   -qf <file>      (run) a pattern file, one edge "a b" per line, instead of
+//        a -q preset; its symmetry-breaking conditions come from the
+//        pattern's own automorphism group
 //   -budget <GB>    device memory per consumer
 //   -json <file>    totals, histogram and per-form timings as JSON
 #pragma once

@@ -248,7 +248,7 @@ def test_host_spill_tiny_budget(g2, orc, cfg1):
     tune = dict(wave_scratch_words=256, big_waves=64)  # most roots above degree ~20 are deferred
     with g2.Executor(device=0, **tune) as ex:
         ex.transfer(g)
-        free = ex.run_mce()
+        free = ex.run_mce(per_root=True)  # the same buffers as the run below
     st = free["stats"]
     need, nbig = st["max_scratch_words"], st["big_roots"]
     assert nbig > 0 and st["host_spill_bytes"] == 0

@@ -113,3 +113,23 @@ def test_bk_aimd_retry(orc, g2, tmp_path):
     assert r2["total"] == ref["total"]
     assert {int(k): v for k, v in r2["hist"].items()} == \
         {int(k): int(ref["hist"][k]) for k in np.nonzero(ref["hist"])[0]}
+
+
+def test_run_pattern_file(g2, orc, binned_graph):
+    """run -qf: a pattern given as an edge list instead of a preset; the total
+    equals the oracle's count of the same plan (and the brute-force count of
+    injective homomorphisms over |Aut| on the whole graph)."""
+    g, path, d = binned_graph
+    qadj = g2.query_random(5, 0.7, 40021)
+    pf = d / "pattern.txt"
+    with open(pf, "w") as f:
+        f.write("# a random connected 5-vertex pattern\n")
+        for a in range(5):
+            for b in range(a + 1, 5):
+                if qadj[a] >> b & 1:
+                    f.write(f"{a} {b}\n")
+    r, log = run_cli("run", ["-dg", path, "-qf", str(pf), "-pn", "2", "-cn", "2", "-qs", "2"], d / "runqf.json")
+    p = g2.plan_build(g2.query_remap(qadj))
+    ref = orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(p)))["count"]
+    assert r["total"] == ref
+    assert f"Total counts: {ref}" in log

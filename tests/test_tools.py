@@ -133,3 +133,15 @@ def test_kmeans_and_view_pack_cli(g2, tmp_path):
     assert len(lines) == g.nv + 1 and {int(x) for x in lines[1:]} == set(range(nb))
     nb2, bins = g2.read_vbmap(str(tmp_path / "p.bin.2.hop.vbmap"), g.nv)
     assert nb2 == nb and bins.max() == nb - 1
+
+
+@pytest.mark.parametrize("tool_name,exts", [("toGAMMA", ("col", "dst", "vlabel")), ("toGSI", ("g",)),
+                                            ("toPangolin", ("pangolin.txt",))])
+def test_converters_byte_identical(tmp_path, tool_name, exts):
+    """toGAMMA / toGSI / toPangolin outputs equal the reference converters'
+    (tests/golden/make_converter_fixtures.py)"""
+    g = tmp_path / "toy_0.bin"
+    shutil.copy(os.path.join(GOLD, "toy_0.pre.bin"), g)
+    tool(tool_name, g)
+    for ext in exts:
+        assert (tmp_path / f"toy_0.{ext}").read_bytes() == open(os.path.join(GOLD, f"toy_0.{ext}"), "rb").read(), ext
