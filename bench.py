@@ -211,93 +211,120 @@ def time_mce(ex, steps, warmup, **kw):
 
 
 def mce_cpu_baseline(orc, g, target_s, seed=7):
-    threads = os.cpu_count() or 1
+    """oracle MCE on a seeded root sample sized to ~target_s seconds per run;
+    one warm-up, then the median of 3"""
+    model, threads = host_cpu()
     rng = np.random.default_rng(seed)
     perm = rng.permutation(g.nv).astype(np.uint32)
     n = max(1, g.nv // 10000)
     while True:
         t0 = time.perf_counter()
-        r = orc.mce(g, sources=np.sort(perm[:n]), threads=threads)
+        orc.mce(g, sources=np.sort(perm[:n]), threads=threads)
         dt = time.perf_counter() - t0
         if dt >= target_s * 0.5 or n == g.nv:
             break
         n = min(g.nv, int(n * max(2.0, target_s / max(dt, 1e-3))))
-    return {"value": r["total"] / dt, "unit": "maximal cliques/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of {g.nv} roots (seeded uniform sample), {r['total']} cliques in {dt:.2f}s"}
+    src = np.sort(perm[:n])
+    r, dt = timed_median(lambda: orc.mce(g, sources=src, threads=threads))
+    return {"value": r["total"] / dt, "unit": "maximal cliques/s", "cores": threads, "kind": "port", "cpu": model,
+            "sample": f"{n} of {g.nv} roots (seeded uniform sample), {r['total']} cliques in {dt:.2f}s "
+                      "(median of 3 after a warm-up)"}
+
+
+def run_cli(name, cli_args, timeout):
+    """Run one of the shipped command lines with -json; its output streams to
+    stderr (progress), the parsed JSON is returned."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.json")
+        cmd = [os.path.join(ROOT, "g2-aimd_amd", name)] + cli_args + ["-progress", "1", "-json", out]
+        log("running " + " ".join(cmd))
+        p = subprocess.run(cmd, stdout=sys.stderr, stderr=sys.stderr, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"{name} exited with {p.returncode}")
+        with open(out) as f:
+            return json.load(f), " ".join([name] + cli_args)
 
 
 def run_social(args, g2, dev):
-    """Maximal cliques on the social graph: the plain form (degeneracy order,
-    one root per wave, no donation) against the learned + batched form (MLP
-    order and heavy-root splits, work donation, lane-group batching of small
-    roots, autotuned launch shape), on one uploaded view."""
-    S = importlib.import_module("g2-aimd_amd.sched")
-    g2.set_progress(True)
-    xs, ys = [], []
-    for i in range(args.train_graphs):
-        gs, cs = social_graph(g2, S, args.train_n, SOCIAL_SEED + 1 + i)
-        x, y, _ = S.logged_rows(dev, gs, cs)
-        xs.append(x)
-        ys.append(y)
-        log(f"training graph {i}: {gs.nv} roots logged")
-        del gs
-    X, Y = np.concatenate(xs), np.concatenate(ys)
-    norm = S.Standardizer(X)
-    mlp = dev.Mlp(device=0, seed=1)
-    mse = mlp.train(norm(X), Y, epochs=args.epochs, batch=1024, lr=2e-3)
-    log(f"cost model: {len(X)} training roots, last-epoch mse {mse:.4f}")
-    g, core = social_graph(g2, S, args.social_n, SOCIAL_SEED)
-    log(f"social graph: V={g.nv} E={g.ne // 2} undirected, max degree {int(g.degrees().max())}")
-    with dev.Executor(device=0, mce_donate_every=NEVER, mce_roots_per_wave=1) as ex:
-        t0 = time.perf_counter()
-        ex.transfer(g)
-        upload_ms = (time.perf_counter() - t0) * 1e3
-        plain, plain_ms, plain_kms, plain_idle = time_mce(ex, args.steps, args.warmup)
-        f = ex.features()
-        x = S.root_features(ex, np.arange(g.nv, dtype=np.uint32), core, g.nv)
-        pred = mlp.infer(norm(x))
-        waves = dev.device_info(0)["num_cus"] * 16
-        order, split = S.learned_schedule(pred, f[:, 4], waves)
-        del f, x
-        shape, _, trials = S.autotune_mce(ex, log=log, order=order, split=split)
-        lrn, lrn_ms, lrn_kms, lrn_idle = time_mce(ex, args.steps, args.warmup, order=order, split=split)
-        if lrn["total"] != plain["total"] or not np.array_equal(lrn["hist"], plain["hist"]):
-            raise RuntimeError("the learned + batched form changed the result")
-        # modelled bytes per clique on a seeded 0.2% root sample (accounting variant)
-        rng = np.random.default_rng(11)
-        perm = rng.permutation(g.nv).astype(np.uint32)
-        ns = max(1, g.nv // 500)
-        ex.set_tuning()
-        acc = ex.run_mce(account=True, order=perm, begin=0, end=ns)
-        per_clique = acc["stats"]["alg_bytes"] / max(acc["total"], 1)
-    mlp.close()
-    total = plain["total"]
-    alg = per_clique * total
-    achieved = alg / (lrn_kms / 1e3) / 1e9
+    """Maximal cliques on the social graph through the drop-in pipeline (the
+    bk command line: 4 kcore bins, 4 producers, 4 consumers, queue 4, sorted
+    sources), the plain schedule (source order, one root per wave, no
+    donation) against the learned + batched one (cost-model order and splits,
+    work donation, lane groups of 32, 24 waves per CU), the cost model trained
+    by bk on plain runs of --train-graphs other seeds."""
+    import tempfile
+    model = os.path.join(tempfile.gettempdir(), f"g2_social_{os.getpid()}.model")
+    train = ",".join(f"social:{args.train_n}:{SOCIAL_SEED + 1 + i:#x}" for i in range(args.train_graphs))
+    cli = ["-gen", f"social:{args.social_n}", "-h", "2", "-pn", "4", "-cn", "4", "-qs", "4", "-ss", "1",
+           "-m", str(args.holder_gb), "-sched", "both", "-train", model, "-train-gen", train,
+           "-epochs", str(args.epochs)]
+    r, cmdline = run_cli("bk", cli, timeout=3000)
+    try:
+        os.remove(model)
+    except OSError:
+        pass
+    fx = headline_fixture(f"social-{args.social_n}")
+    check = None
+    if fx is not None:
+        if r["total"] != fx["max_cliques"] or {int(k): v for k, v in r["hist"].items()} != \
+                {int(k): v for k, v in fx["hist"].items()}:
+            raise RuntimeError("maximal cliques differ from the oracle's (tests/golden/headline_counts.json)")
+        check = "total and histogram equal to the oracle's (tests/golden/headline_counts.json)"
+    pl, lr = r["plain"], r["learned"]
+    total = r["total"]
     out = {
-        "metric": "maximal cliques/s (learned + batched form)", "value": total / (lrn_ms / 1e3),
-        "unit": "maximal cliques/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": lrn_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": "maximal cliques/s (learned + batched form, bk pipeline)",
+        "value": total / lr["run_seconds"], "unit": "maximal cliques/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": lr["run_seconds"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32 vertex ids, u64 bitsets and counts", "data": "synthetic (seeded social generator)",
-        "config": {"workload": f"maximal clique enumeration on the social graph, n={args.social_n}",
-                   "vertices": g.nv, "undirected_edges": g.ne // 2, "max_cliques": total,
-                   "histogram_nonzero": {int(k): int(v) for k, v in enumerate(plain["hist"]) if v}},
-        "plain": {"value": total / (plain_ms / 1e3), "ms_per_step": plain_ms, "kernel_ms": plain_kms,
-                  "wave_idle_frac": plain_idle},
-        "learned_batched": {"kernel_ms": lrn_kms, "wave_idle_frac": lrn_idle, "shape": shape,
-                            "tuner_trials": trials, "split_roots": int((split > 1).sum())},
-        "speedup_vs_plain": plain_ms / lrn_ms,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": alg,
-                     "alg_bytes_source": f"{per_clique:.1f} modelled bytes/clique on {ns} sampled roots x "
-                                         f"{total} cliques",
-                     "plain_achieved": alg / (plain_kms / 1e3) / 1e9},
-        "upload_ms": upload_ms,
+        "config": {"workload": f"maximal clique enumeration on the social graph, n={args.social_n}, through "
+                               "the bk pipeline (4 kcore bins, -pn 4 -cn 4 -qs 4 -ss 1)",
+                   "command": cmdline, "vertices": r["vertices"], "undirected_edges": r["edges"],
+                   "max_cliques": total, "check": check,
+                   "histogram_nonzero": {int(k): int(v) for k, v in r["hist"].items()}},
+        "plain": {"value": total / pl["run_seconds"], "run_seconds": pl["run_seconds"],
+                  "pipeline_seconds": pl["seconds"], "kernel_ms": pl["kernel_ms"],
+                  "wave_idle_frac": pl["wave_idle_frac"], "consumer_kernel_ms": pl["consumer_kernel_ms"]},
+        "learned_batched": {"run_seconds": lr["run_seconds"], "pipeline_seconds": lr["seconds"],
+                            "kernel_ms": lr["kernel_ms"], "schedule_ms": lr["schedule_ms"],
+                            "wave_idle_frac": lr["wave_idle_frac"], "donations": lr["donations"],
+                            "split_roots": lr["split_roots"], "consumer_kernel_ms": lr["consumer_kernel_ms"]},
+        "speedup_vs_plain": pl["run_seconds"] / lr["run_seconds"],
+        "note": "run_seconds = wall time during which a consumer was running a view (features, inference and "
+                "the schedule included; materialisation and uploads, the same in both forms, excluded); the "
+                "four consumers share the GPU",
     }
-    if not args.no_cpu:
-        import oracle
-        out["cpu_baseline"] = mce_cpu_baseline(oracle, g, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def run_views(args, g2, dev):
+    """4-cliques on R-MAT through the drop-in view-bin pipeline: the run
+    command line over 4 k-means views (the reference's binning for this
+    workload), 4 producers, 4 consumers, queue 4."""
+    cli = ["-gen", f"rmat:{args.scale}", "-bins", "kmeans", "-nb", "4", "-q", "24", "-pn", "4", "-cn", "4",
+           "-qs", "4", "-m", str(args.holder_gb)]
+    r, cmdline = run_cli("run", cli, timeout=2400)
+    pl = r["plain"]
+    fx = headline_fixture(f"rmat-{args.scale}")
+    check = None
+    if fx is not None and "q24" in fx:
+        if r["total"] != fx["q24"] or 2 * r["edges"] != fx["ne"]:
+            raise RuntimeError("4-clique total differs from the oracle's (tests/golden/headline_counts.json)")
+        check = "equal to the oracle's total (tests/golden/headline_counts.json)"
+    out = {
+        "metric": "4-clique matches/s through 4 k-means view bins", "value": r["total"] / pl["run_seconds"],
+        "unit": "4-clique matches/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": pl["run_seconds"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 vertex ids, u64 bitsets and counts", "data": "synthetic (seeded R-MAT)",
+        "config": {"workload": f"4-clique matching (q=24) on R-MAT scale {args.scale} through the run pipeline "
+                               "(4 k-means view bins, -pn 4 -cn 4 -qs 4)", "command": cmdline,
+                   "vertices": r["vertices"], "undirected_edges": r["edges"], "matches": r["total"],
+                   "check": check},
+        "run_seconds": pl["run_seconds"], "pipeline_seconds": pl["seconds"], "kernel_ms": pl["kernel_ms"],
+        "consumer_kernel_ms": pl["consumer_kernel_ms"],
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -323,6 +350,9 @@ def run_mce_plain(args, g2, dev):
                                   + (f", roots of degree {args.source_degree}" if args.source_degree else ""),
                       "roots": int(g.nv if src is None else len(src)), "max_cliques": r["total"],
                       "nodes": r["stats"]["nodes"], "kernel_ms": kms, "wave_idle_frac": idle}}
+    if not args.no_cpu:
+        import oracle
+        out["cpu_baseline"] = mce_cpu_baseline(oracle, g, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
@@ -450,9 +480,10 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MB "
                          "(rocprof counter calibration) and exit")
-    ap.add_argument("--workload", choices=("matching", "social", "batch", "mce"), default="matching",
+    ap.add_argument("--workload", choices=("matching", "social", "views", "batch", "mce"), default="matching",
                     help="matching: 4-clique matching on R-MAT (default line); social: maximal "
-                         "cliques on the social graph, plain vs learned + batched; batch: many "
+                         "cliques on the social graph through bk, plain vs learned + batched; views: "
+                         "4-cliques through run over 4 k-means view bins; batch: many "
                          "random patterns, batched vs serial launches (use --scale 24); mce: "
                          "plain maximal cliques on a social graph (--social-n, --source-degree)")
     ap.add_argument("--queries", type=int, default=512)
@@ -464,10 +495,12 @@ def main():
     ap.add_argument("--train-n", type=int, default=2_000_000)
     ap.add_argument("--train-graphs", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--holder-gb", type=float, default=16.0,
+                    help="social/views: pinned view-holder size (-m of the command lines)")
     args = ap.parse_args()
-    if args.workload in ("social", "batch", "mce") and args.gpus == 1:
+    if args.workload in ("social", "views", "batch", "mce") and args.gpus == 1:
         g2 = importlib.import_module("g2-aimd_amd")
-        fn = {"social": run_social, "batch": run_batch, "mce": run_mce_plain}[args.workload]
+        fn = {"social": run_social, "views": run_views, "batch": run_batch, "mce": run_mce_plain}[args.workload]
         fn(args, g2, importlib.import_module("g2-aimd_amd.device"))
         return
     if args.copy_calibration_mb:

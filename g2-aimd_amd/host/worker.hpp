@@ -189,6 +189,7 @@ struct RunResult {
     uint64_t host_spill_bytes = 0;      // largest host spill of a launch
     uint32_t local_views = 0;           // GM: views counted by the local-bitmap path
     uint64_t device_bytes = 0;          // largest device memory held after a launch
+    std::vector<std::pair<double, double>> spans;  // wall-clock (us) of each view's run, schedule included
     void add(const RunResult& r);
 };
 
