@@ -7,6 +7,8 @@
 //   (the size of the root's P set in the maximal-clique search tree);
 //   triangles are counted by streaming the shorter of N(v), N(u) and binary
 //   searching the other, one neighbour u at a time.
+// * core_step_kernel: core numbers by the h-index iteration;
+//   inputs_kernel: the cost model's 8 inputs in fp32.
 // * mlp: 8 -> 64 -> 64 -> 1 with ReLU, BF16 weights and activations on the
 //   matrix cores (v_mfma_f32_16x16x16bf16_1k), FP32 accumulation, FP32 master
 //   weights and Adam.  A wave owns 16 samples: layer 1 is 4 MFMA tiles (K padded
@@ -80,6 +82,104 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
             o[3] = (int64_t)shared;
             o[4] = (int64_t)p0;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// core numbers: the h-index iteration (Lu, Zhou, Zhang, Stanley, "The H-index
+// of a network node and its relation to degree and coreness", Nature
+// Communications 7, 2016): starting from the degrees, each vertex's value
+// becomes the h-index of its neighbours' values (the largest k with at least
+// k neighbours valued >= k) until nothing changes; the values only decrease,
+// stay >= the core number, and end at it.  Updates are applied in place
+// (asynchronously); a vertex is revisited only when a neighbour decreased.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t count_at_least(const uint32_t* __restrict__ cols, uint64_t lo, uint64_t hi,
+                                          const uint32_t* h, uint32_t k) {
+    uint32_t c = 0;
+    for (uint64_t e = lo + lane_id(); e < hi; e += kWave) c += __hip_atomic_load(&h[cols[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= k;
+    return (uint32_t)wave_sum(c);
+}
+
+__global__ __launch_bounds__(256) void core_init_kernel(const uint64_t* __restrict__ rp, uint64_t nv, uint32_t* h,
+                                                        uint8_t* act) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        h[v] = (uint32_t)min<uint64_t>(rp[v + 1] - rp[v], 0xffffffffull);
+        act[v] = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void core_clear_kernel(uint8_t* act, uint64_t nv) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x)
+        act[v] = 0;
+}
+
+__global__ __launch_bounds__(256) void core_step_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols,
+                                                        uint64_t nv, uint32_t* h, const uint8_t* __restrict__ act_in,
+                                                        uint8_t* act_out, unsigned long long* changed) {
+    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / kWave;
+    for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < nv; v += nw) {
+        if (!act_in[v]) continue;
+        const uint32_t cur = __hip_atomic_load(&h[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == 0) continue;
+        const uint64_t lo = rp[v], hi = rp[v + 1];
+        if (count_at_least(cols, lo, hi, h, cur) >= cur) continue;
+        uint32_t a = 0, b = cur - 1;  // k = 0 always holds, k = cur does not
+        while (a < b) {
+            const uint32_t mid = a + (b - a + 1) / 2;
+            if (count_at_least(cols, lo, hi, h, mid) >= mid) a = mid;
+            else b = mid - 1;
+        }
+        if (lane_id() == 0) {
+            __hip_atomic_store(&h[v], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicOr(changed, 1ull);
+        }
+        for (uint64_t e = lo + lane_id(); e < hi; e += kWave) act_out[cols[e]] = 1;
+    }
+}
+
+// core numbers of the uploaded view's vertices into dcore (device, nv u32)
+static int core_numbers_run(g2_ctx* c, uint32_t* dcore, uint8_t* act0, uint8_t* act1) {
+    hipError_t e;
+    const uint64_t nv = c->nv;
+    const auto* rp = (const uint64_t*)c->rp.ptr;
+    const auto* cols = (const uint32_t*)c->cols.ptr;
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((nv + 255) / 256, 8192));
+    const uint64_t wgrid = std::max<uint64_t>(1, (uint64_t)c->num_cus * 32 / 4);
+    auto* changed = (unsigned long long*)c->counters.ptr;
+    hipLaunchKernelGGL(core_init_kernel, dim3(grid), dim3(256), 0, c->stream, rp, nv, dcore, act0);
+    for (int it = 0;; it++) {
+        hipLaunchKernelGGL(core_clear_kernel, dim3(grid), dim3(256), 0, c->stream, act1, nv);
+        if ((e = hipMemsetAsync(changed, 0, 8, c->stream))) return hip_err(c, e, "memset");
+        hipLaunchKernelGGL(core_step_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, nv, dcore, act0, act1, changed);
+        if ((e = hipGetLastError())) return hip_err(c, e, "core launch");
+        unsigned long long h = 0;
+        if ((e = hipMemcpyAsync(&h, changed, 8, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H core");
+        if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "core sync");
+        if (!h) break;
+        std::swap(act0, act1);
+    }
+    return G2_OK;
+}
+
+// The cost model's 8 inputs in fp32, from the integer features, the core
+// numbers and the source ids (the fp32 formula, one rounding per operation)
+__global__ __launch_bounds__(256) void inputs_kernel(const int64_t* __restrict__ feat, const uint32_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ core, uint64_t ns, uint64_t nv,
+                                                     float* __restrict__ x) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int64_t* f = feat + i * G2_NFEAT_INT;
+        const float d = (float)f[0], tri = (float)f[2];
+        const float pairs = (float)(f[0] * (f[0] - 1));
+        float* o = x + i * 8;
+        o[0] = log1pf(d);
+        o[1] = log1pf((float)core[src[i]]);
+        o[2] = log1pf((float)f[1]);
+        o[3] = log1pf(tri);
+        o[4] = f[0] > 1 ? (2.0f * tri) / pairs : 0.0f;
+        o[5] = (float)src[i] / (float)(nv ? nv : 1);
+        o[6] = (float)f[3] / fmaxf(d, 1.0f);
+        o[7] = log1pf((float)f[4]);
     }
 }
 
@@ -484,6 +584,72 @@ int g2_features(g2_ctx* c, int64_t* out) {
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
     return features_run(c, out);
+}
+
+int g2_core_numbers(g2_ctx* c, uint32_t* core) {
+    if (!c || !core) return G2_EINVAL;
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    int rc;
+    if ((rc = ensure(c, c->counters, 8, "counters"))) return rc;
+    struct CallBuf {
+        g2_ctx* c;
+        DevBuf b;
+        ~CallBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+        }
+    } buf{c, {}};
+    const uint64_t nv = std::max<uint64_t>(c->nv, 1);
+    if ((rc = ensure(c, buf.b, nv * 4 + 2 * ((nv + 255) & ~255ull), "core numbers"))) return rc;
+    auto* dcore = (uint32_t*)buf.b.ptr;
+    auto* act = (uint8_t*)buf.b.ptr + nv * 4;
+    if ((rc = core_numbers_run(c, dcore, act, act + ((nv + 255) & ~255ull)))) return rc;
+    if (c->nv && (e = hipMemcpy(core, dcore, c->nv * 4, hipMemcpyDeviceToHost))) return hip_err(c, e, "D2H core");
+    return G2_OK;
+}
+
+int g2_root_inputs(g2_ctx* c, const uint32_t* core, float* out) {
+    if (!c || !out) return G2_EINVAL;
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    int rc;
+    if ((rc = ensure(c, c->counters, 8, "counters"))) return rc;
+    struct CallBuf {
+        g2_ctx* c;
+        DevBuf b;
+        ~CallBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+        }
+    } buf{c, {}};
+    const uint64_t ns = std::max<uint64_t>(c->ns, 1), nv = std::max<uint64_t>(c->nv, 1);
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    const uint64_t o_x = al(ns * G2_NFEAT_INT * 8), o_core = o_x + al(ns * 8 * 4), o_act = o_core + al(nv * 4);
+    if ((rc = ensure(c, buf.b, o_act + 2 * al(nv), "root inputs"))) return rc;
+    char* base = (char*)buf.b.ptr;
+    auto* feat = (int64_t*)base;
+    auto* x = (float*)(base + o_x);
+    auto* dcore = (uint32_t*)(base + o_core);
+    if (core) {
+        if ((e = hipMemcpyAsync(dcore, core, c->nv * 4, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D core");
+    } else if ((rc = core_numbers_run(c, dcore, (uint8_t*)(base + o_act), (uint8_t*)(base + o_act + al(nv))))) {
+        return rc;
+    }
+    if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
+    FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
+               (unsigned long long*)c->counters.ptr, feat};
+    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
+    hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((c->ns + 255) / 256, 8192));
+    hipLaunchKernelGGL(inputs_kernel, dim3(grid), dim3(256), 0, c->stream, feat, (const uint32_t*)c->src.ptr, dcore,
+                       c->ns, c->nv, x);
+    if ((e = hipGetLastError())) return hip_err(c, e, "root inputs launch");
+    if (c->ns && (e = hipMemcpyAsync(out, x, c->ns * 8 * 4, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H inputs");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "root inputs sync");
+    return G2_OK;
 }
 
 int g2_mlp_create(g2_mlp** out, int device, uint64_t seed) {

@@ -29,7 +29,7 @@ EXPORTS = ("g2_create", "g2_set_tuning", "g2_destroy", "g2_last_error", "g2_uplo
            "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
            "g2_pinned_free", "g2_plan_local_eligible", "g2_features", "g2_mlp_create",
            "g2_mlp_destroy", "g2_mlp_get_params", "g2_mlp_set_params", "g2_mlp_infer", "g2_mlp_grad",
-           "g2_mlp_train", "g2_gm_run_batch", "g2_gm_autotune")
+           "g2_mlp_train", "g2_gm_run_batch", "g2_gm_autotune", "g2_core_numbers", "g2_root_inputs")
 G2_NFEAT_INT = 5
 G2_MLP_NPARAM = 4801
 
@@ -102,6 +102,8 @@ def hip_lib() -> C.CDLL:
                                  C.POINTER(Batch), C.POINTER(C.c_double)]
     f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.g2_features.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    L.g2_core_numbers.argtypes = [vp, u32p]
+    L.g2_root_inputs.argtypes = [vp, vp, f32p]
     L.g2_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
     L.g2_mlp_destroy.argtypes = [vp]
     L.g2_mlp_destroy.restype = None
@@ -143,6 +145,7 @@ class Executor:
         if rc != 0:
             raise RuntimeError(f"g2_create failed ({rc}): no usable GPU or bad arguments")
         self.ns = 0
+        self.nv = 0
 
     def set_tuning(self, **tuning) -> None:
         """New launch tuning for the next runs (the view stays uploaded)."""
@@ -180,6 +183,7 @@ class Executor:
         self._check(hip_lib().g2_upload_view(self._ctx, g.row_ptrs, g.nv, cols, g.ne, s, len(src)),
                     "g2_upload_view")
         self.ns = len(src)
+        self.nv = g.nv
 
     def run_mce(self, hist_len: int = 256, per_root: bool = False, order=None, split=None,
                 account: bool = False, begin: int = 0, end: int = 0) -> dict:
@@ -275,6 +279,20 @@ class Executor:
         out = b.to_dict()
         out["sample_ms"] = ms.value
         return out
+
+    def core_numbers(self) -> np.ndarray:
+        """core number of every vertex of the uploaded view (device h-index iteration)"""
+        out = np.zeros(max(self.nv, 1), np.uint32)
+        self._check(hip_lib().g2_core_numbers(self._ctx, out), "g2_core_numbers")
+        return out[: self.nv]
+
+    def root_inputs(self, core=None) -> np.ndarray:
+        """the cost model's 8 fp32 inputs of every source, computed on the device"""
+        out = np.zeros((max(self.ns, 1), 8), np.float32)
+        cr = None if core is None else np.ascontiguousarray(core, dtype=np.uint32)
+        self._check(hip_lib().g2_root_inputs(self._ctx, cr.ctypes.data if cr is not None else None, out),
+                    "g2_root_inputs")
+        return out[: self.ns]
 
     def features(self) -> np.ndarray:
         """Integer root features (ns x 5): degree, 2-paths, triangles,

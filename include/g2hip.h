@@ -197,6 +197,19 @@ int  g2_plan_local_eligible(const g2_plan* plan);
 #define G2_NFEAT_INT 5
 int  g2_features(g2_ctx* ctx, int64_t* out);
 
+/* Core number of every vertex of the uploaded view (nv entries), by the
+ * h-index iteration on the device; equal to the graph's core numbers for a
+ * whole-graph view (a partial view's rows give its own). */
+int  g2_core_numbers(g2_ctx* ctx, uint32_t* core);
+/* The cost model's 8 inputs of every uploaded source (ns x 8 fp32, source
+ * order), computed on the device with one fp32 rounding per operation:
+ * log1p(degree), log1p(core number), log1p(2-paths), log1p(triangles),
+ * local clustering 2t / (d(d-1)), id / nv (degeneracy rank of a recoded
+ * graph), shared-neighbour fraction, log1p(later neighbours).  core (host,
+ * nv entries) gives the core numbers; NULL computes them from the view
+ * (g2_core_numbers).  g2h_root_features is the same formula on the host. */
+int  g2_root_inputs(g2_ctx* ctx, const uint32_t* core, float* out);
+
 /* Cost model: MLP 8 -> 64 -> 64 -> 1 (ReLU), BF16 weights/activations on the
  * matrix cores, FP32 accumulation, FP32 master weights, Adam.  Parameters
  * (fp32): W1[8][64], b1[64], W2[64][64], b2[64], W3[64], b3. */

@@ -194,3 +194,41 @@ def test_mlp_training_is_reproducible(dev):
         m.close()
     assert np.array_equal(params[0], params[1])
     assert grads[0][0] == grads[1][0] and np.array_equal(grads[0][1], grads[1][1])
+
+
+def _ulps(a, b):
+    """distance in units in the last place between two float32 arrays"""
+    ia = a.astype(np.float32).view(np.int32).astype(np.int64)
+    ib = b.astype(np.float32).view(np.int32).astype(np.int64)
+    return np.abs(ia - ib)
+
+
+def test_device_core_numbers_and_inputs(g2, dev):
+    """Core numbers from the device's h-index iteration equal the host bucket
+    peel; the 8 fp32 model inputs computed on the device are within 1 ulp of
+    the same fp32 formula on the CPU (NumPy float32, one rounding per step)."""
+    print("case: device core numbers and model inputs", flush=True)
+    import graphs as G
+    S = importlib.import_module("g2-aimd_amd.sched")
+    cases = [S.prepare_graph(g2, 13, 0x6732A1D0C0002001)[0], social(g2, S, 200_000, 0x6732A1D0C0000310)[0],
+             G.complete(g2, 12), G.cycle(g2, 9), G.complete_bipartite(g2, 3, 7)]
+    for g in cases:
+        with dev.Executor(device=0) as ex:
+            ex.transfer(g)
+            core = ex.core_numbers()
+            assert np.array_equal(core, S.core_numbers_csr(g))
+            x = ex.root_inputs()
+            assert np.array_equal(x, ex.root_inputs(core))
+            f = ex.features()
+        f32 = np.float32
+        d = f[:, 0].astype(f32)
+        tri = f[:, 2].astype(f32)
+        pairs = (f[:, 0] * (f[:, 0] - 1)).astype(f32)
+        src = np.arange(g.nv)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            clust = np.where(f[:, 0] > 1, (f32(2) * tri) / pairs, f32(0))
+        ref = np.stack([np.log1p(d), np.log1p(core.astype(f32)), np.log1p(f[:, 1].astype(f32)), np.log1p(tri),
+                        clust.astype(f32), src.astype(f32) / f32(max(g.nv, 1)),
+                        f[:, 3].astype(f32) / np.maximum(d, f32(1)), np.log1p(f[:, 4].astype(f32))], axis=1)
+        assert ref.dtype == np.float32
+        assert int(_ulps(x, ref).max()) <= 1
