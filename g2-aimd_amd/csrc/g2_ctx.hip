@@ -199,6 +199,7 @@ int g2_upload_view(g2_ctx* c, const uint64_t* rp, uint64_t nv, const uint32_t* c
     // From here on a failure leaves no view: runs return G2_ENOVIEW until an
     // upload succeeds.
     c->view_ok = false;
+    c->orient_valid = false;
     c->nv = c->ne = c->ns = 0;
     auto fail = [&](int rc) {
         release(c, c->rp);

@@ -26,7 +26,8 @@
 // reference's per-bin partial counts, which root each match at its
 // lowest-id vertex, are split differently; their sum is the same).
 //
-// Each run first builds the oriented relabelled CSR on the device:
+// The first run on an uploaded view builds the oriented relabelled CSR on the
+// device (later runs on the same view reuse it):
 //   pi      = rank of (degree, id) over all vertices (radix sort),
 //   N+(x)   = { pi(w) : w in N(v), pi(w) > x } for x = pi(v), sorted.
 // U(x) is then the row N+(x) itself, already in rank order, and the rows of
@@ -850,6 +851,9 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
     int rc;
     if ((rc = ensure(c, c->orient, total, "oriented rows"))) return rc;
     char* base = (char*)c->orient.ptr;
+    // the rows depend on the view only: built once per upload, kept for
+    // later runs (any plan) until the next upload
+    const bool cached = c->orient_valid;
     auto* keys = (unsigned long long*)base;
     auto* sorted = (unsigned long long*)(base + o_sorted);
     auto* cnt = (unsigned long long*)(base + o_cnt);
@@ -861,6 +865,12 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
     const auto* cols = (const uint32_t*)c->cols.ptr;
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((nv + 255) / 256, 8192));
     const uint64_t wgrid = (uint64_t)c->num_cus * 32 / 4;  // 32 waves per CU: latency-bound row walks
+    if (cached) {
+        *orp = (const uint64_t*)d_orp;
+        *ocols = d_ocols;
+        *rank = pi;
+        return G2_OK;
+    }
     hipLaunchKernelGGL(okey_kernel, dim3(grid), dim3(256), 0, c->stream, (const uint32_t*)c->deg.ptr, nv, keys);
     size_t tb = t_sort;
     if ((e = hipcub::DeviceRadixSort::SortKeys(base + o_temp, tb, keys, sorted, (int)nv, 0, nbits, c->stream)))
@@ -876,6 +886,7 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
     *orp = (const uint64_t*)d_orp;
     *ocols = d_ocols;
     *rank = pi;
+    c->orient_valid = true;
     return G2_OK;
 }
 

@@ -112,6 +112,7 @@ struct g2_ctx {
     g2::LocalProgram lprog{};
     g2::DevBuf big_list2;
     g2::DevBuf orient;            // oriented relabelled rows of the local-bitmap path
+    bool orient_valid = false;    // orient holds the rows of the current view
     g2::DevBuf roots;             // local-bitmap path: ranks of the sources of a partial view
     g2_plan plan{};
 
