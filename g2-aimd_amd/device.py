@@ -29,7 +29,8 @@ EXPORTS = ("g2_create", "g2_set_tuning", "g2_destroy", "g2_last_error", "g2_uplo
            "g2_gm_set_plan", "g2_gm_run", "g2_device_info", "g2_device_count", "g2_pinned_alloc",
            "g2_pinned_free", "g2_plan_local_eligible", "g2_features", "g2_mlp_create",
            "g2_mlp_destroy", "g2_mlp_get_params", "g2_mlp_set_params", "g2_mlp_infer", "g2_mlp_grad",
-           "g2_mlp_train", "g2_gm_run_batch", "g2_gm_autotune", "g2_core_numbers", "g2_root_inputs")
+           "g2_mlp_train", "g2_gm_run_batch", "g2_gm_autotune", "g2_core_numbers", "g2_root_inputs",
+           "g2_gm_run_noalloc", "g2_noalloc_last_error")
 G2_NFEAT_INT = 5
 G2_MLP_NPARAM = 4801
 

@@ -147,6 +147,29 @@ int  g2_gm_set_plan(g2_ctx* ctx, const g2_plan* plan);
 int  g2_gm_run(g2_ctx* ctx, uint32_t flags, uint64_t* count, uint64_t* per_root_count,
                g2_stats* stats);
 
+/* ---- allocation-free matching --------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): g2_gm_run's count of `plan` rooted at `sources`, over a
+ * view the caller already holds in device memory (row_ptrs u64[nv+1],
+ * cols u32[ne], sources u32[ns], all device pointers, rows sorted), by the
+ * canonical traversal (G2_RUN_CANONICAL).  *count is a host pointer.
+ * Nothing is allocated: counters and per-wave candidate lists live in a
+ * 64 MiB array of the library's own that every device holds until the
+ * process ends.  Per wave that takes 8 * n * max(max degree, 64) bytes, so
+ * the wave count shrinks with the largest degree; G2_EBUDGET when one wave's
+ * lists do not fit (max degree above about 2^23 / n: 2,097,144 for a
+ * 4-vertex plan).  G2_EINVAL when the plan is
+ * invalid or the CSR is malformed (checked on the device before the search).
+ * Runs on the device's null stream and returns when it has finished: the
+ * caller's writes to the inputs must be complete before the call.  Calls
+ * that overlap on one device share the array and race; calls on different
+ * devices do not.  g2_noalloc_last_error() gives the calling thread's last
+ * error message. */
+int  g2_gm_run_noalloc(int device, const uint64_t* row_ptrs, uint64_t nv, const uint32_t* cols, uint64_t ne,
+                       const uint32_t* sources, uint64_t ns, const g2_plan* plan, uint64_t* count);
+const char* g2_noalloc_last_error(void);
+
 /* ---- batched matching (no counterpart in the reference: its run binary
  * takes one query per process, app_gmatch/main.cu, system/worker.h:28-280) */
 
