@@ -368,11 +368,12 @@ def view_materialize(g: Graph, sources: np.ndarray, hop: int, root_degree: int =
 from .device import (  # noqa: E402  (device side needs the helpers above)
     Executor,
     hip_lib,
+    plan_local_eligible,
 )
 
 __all__ = [
     "G2Plan", "Graph", "Executor", "hip_lib", "host_lib", "gen_chung_lu", "gen_rmat",
     "gen_social", "kcore_bins", "write_vbmap", "read_vbmap", "query_preset", "query_remap",
     "query_conditions", "query_automorphisms", "query_random", "plan_build", "plan_for_preset",
-    "view_materialize",
+    "view_materialize", "plan_local_eligible",
 ]

@@ -154,7 +154,9 @@ __device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint3
 // (looked up in N+(U_i), global memory).
 // Each lane keeps kItemsW items in flight; U is searched by a fixed number
 // of halving steps so the items' loads overlap.  rows and U are LDS or
-// global memory (fixed per call site after inlining).
+// global memory (fixed per call site after inlining).  CLQ (clique plans)
+// sets only the later member's bit: row i then holds the members above i.
+template <bool CLQ>
 __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* U, uint64_t* rows, uint32_t m,
                                             uint32_t W, uint32_t a0) {
     const uint32_t lane = lane_id();
@@ -269,7 +271,41 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             }
             if (hit) {
                 atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
-                atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                if constexpr (!CLQ)
+                    atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+            }
+        }
+    }
+}
+
+// Clique plans (position i adjacent to every earlier one and above i-1):
+// with rows that hold only later members, the candidates of position L are
+// the AND of the matched members' rows, from the word of the last one on --
+// no order range, no holes.  off[1..L-1] are the matched rows' word offsets,
+// last the rank at position L-1, nw the words that hold bits.
+template <int N, int L>
+__device__ __forceinline__ void clique_walk(const uint64_t* rows, uint32_t W, uint32_t nw, uint32_t* off, uint32_t last,
+                                            uint64_t& lane_total, uint64_t& nodes) {
+    if constexpr (L >= N - 1) {
+        uint32_t c = 0;
+        for (uint32_t w = last >> 6; w < nw; w++) {
+            uint64_t x = rows[off[1] + w];
+#pragma unroll
+            for (int j = 2; j < L; j++) x &= rows[off[j] + w];
+            c += (uint32_t)__popcll(x);
+        }
+        lane_total += c;
+    } else {
+        for (uint32_t w = last >> 6; w < nw; w++) {
+            uint64_t x = rows[off[1] + w];
+#pragma unroll
+            for (int j = 2; j < L; j++) x &= rows[off[j] + w];
+            while (x) {
+                const uint32_t b = w * 64 + (uint32_t)__builtin_ctzll(x);
+                x &= x - 1;
+                nodes++;
+                off[L] = b * W;
+                clique_walk<N, L + 1>(rows, W, nw, off, b, lane_total, nodes);
             }
         }
     }
@@ -330,7 +366,7 @@ __device__ __forceinline__ void walk_from(const LocalProgram& P, uint32_t* M, ui
 
 // One root on one wave: build the local rows over U, then each lane walks the
 // matches of the position-1 candidates lane, lane+64, ...
-template <int N>
+template <int N, bool CLQ>
 __device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint32_t m, uint64_t* rows,
                                            const uint32_t* U, uint32_t* Ucopy, uint64_t& lane_total,
                                            uint64_t& nodes) {
@@ -341,14 +377,24 @@ __device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint
         for (uint32_t i = lane; i < m; i += kWave) Ucopy[i] = a.ocols[lo + i];
     wave_sync();
     for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) {
-        build_chunk(a, U, rows, m, W, a0);
+        build_chunk<CLQ>(a, U, rows, m, W, a0);
         wave_sync();  // (the chunk's atomics complete before the walk reads rows)
     }
-    uint32_t M[N];
-    for (uint32_t r1 = lane; r1 < m; r1 += kWave) {
-        M[1] = r1;
-        nodes++;
-        walk_from<N>(a.prog, M, 2, rows, W, m, lane_total, nodes);
+    if constexpr (CLQ) {
+        const uint32_t nw = (m + 63) >> 6;
+        uint32_t off[N];
+        for (uint32_t r1 = lane; r1 < m; r1 += kWave) {
+            nodes++;
+            off[1] = r1 * W;
+            clique_walk<N, 2>(rows, W, nw, off, r1, lane_total, nodes);
+        }
+    } else {
+        uint32_t M[N];
+        for (uint32_t r1 = lane; r1 < m; r1 += kWave) {
+            M[1] = r1;
+            nodes++;
+            walk_from<N>(a.prog, M, 2, rows, W, m, lane_total, nodes);
+        }
     }
 }
 
@@ -366,7 +412,7 @@ __device__ inline void defer_root(const LocalArgs& a, uint32_t x, uint64_t need)
 // Pass 1 (big_list set): roots whose rows fit the wave's LDS; the others are
 // deferred.  Pass 3 (big_list null): the roots too large for a block's LDS,
 // rows in per-wave global arenas of scratch_words.
-template <int N>
+template <int N, bool CLQ>
 __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalArgs a) {
     __shared__ uint64_t lrows[kLocalBlockWaves][kLRowWords];
     __shared__ uint32_t lU[kLocalBlockWaves][kLU];
@@ -398,9 +444,9 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
         } else {
             const uint64_t need = local_need(m);
             if (m <= kLU && need <= kLRowWords) {
-                local_root<N>(a, lo, m, lrows[wid], lU[wid], lU[wid], lane_total, nodes);
+                local_root<N, CLQ>(a, lo, m, lrows[wid], lU[wid], lU[wid], lane_total, nodes);
             } else if (!a.big_list && need <= a.scratch_words) {
-                local_root<N>(a, lo, m, ws, a.ocols + lo, nullptr, lane_total, nodes);
+                local_root<N, CLQ>(a, lo, m, ws, a.ocols + lo, nullptr, lane_total, nodes);
             } else if (lane == 0) {
                 defer_root(a, x, need);
             }
@@ -423,7 +469,7 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
 // position-1 candidates r1 one at a time from an LDS counter, a wave's lanes
 // share the position-2 candidates of its r1, queued 64 at a time in LDS, and
 // each lane walks the rest of its (r1, r2) subtree.
-template <int N, int BT, int ROWW, int BU, int NB>
+template <int N, bool CLQ, int BT, int ROWW, int BU, int NB>
 __global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
     constexpr int kWaves = BT / kWave;
     constexpr int kMembersPerThread = (BU + BT - 1) / BT;
@@ -603,12 +649,57 @@ __global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
                 }
                 if (hit) {
                     atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
-                    atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                    if constexpr (!CLQ)
+                        atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
                 }
             }
         }
         __syncthreads();
 
+        if constexpr (CLQ) {
+            // position-1 members r1 one at a time per wave; the (r1, r2)
+            // pairs of successive r1 share the wave's queue, 64 at a time
+            const uint32_t nw = (m + 63) >> 6;
+            uint32_t k = 0;
+            auto drain_pairs = [&](uint32_t nq) {
+                wave_sync();
+                if (lane < nq) {
+                    const uint32_t e = queue[wid][lane];
+                    uint32_t off[N];
+                    off[1] = (e >> 16) * W;
+                    off[2] = (e & 0xffffu) * W;
+                    nodes++;
+                    clique_walk<N, 3>(rows, W, nw, off, e & 0xffffu, lane_total, nodes);
+                }
+                wave_sync();
+            };
+            while (true) {
+                __builtin_amdgcn_wave_barrier();
+                uint32_t r1 = 0;
+                if (lane == 0) r1 = atomicAdd(&s_next, 1u);
+                r1 = uniform(r1);
+                if (r1 >= m) break;
+                if (lane == 0) nodes++;
+                if constexpr (N == 3) {
+                    for (uint32_t w = (r1 >> 6) + lane; w < nw; w += kWave) lane_total += __popcll(rows[r1 * W + w]);
+                } else {
+                    for (uint32_t w = r1 >> 6; w < nw; w++) {
+                        const uint64_t cw = rows[r1 * W + w];
+                        const uint32_t cnt = (uint32_t)__popcll(cw);
+                        if (k + cnt > kWave) {
+                            drain_pairs(k);
+                            k = 0;
+                        }
+                        if ((cw >> lane) & 1) queue[wid][k + lanes_below(cw)] = (r1 << 16) | (w * 64 + lane);
+                        k += cnt;
+                    }
+                }
+            }
+            if constexpr (N > 3) {
+                if (k) drain_pairs(k);
+            }
+            continue;
+        }
         uint32_t M[N];
         while (true) {
             __builtin_amdgcn_wave_barrier();
@@ -958,33 +1049,39 @@ void gm_local_compile(g2_ctx* c, const g2_plan* p) {
         lt.push_back({0, i});
     }
     if (!complete_symmetry_breaking(n, adj, lt)) return;
+    // clique plans: position i >= 2 adjacent to all of 1..i-1 and above i-1
+    // (NON_EQUAL between adjacent positions always holds)
+    bool clq = n >= 3;
+    for (uint32_t i = 2; i < n; i++)
+        clq = clq && L.bn_mask[i] == (1u << i) - 2 && ((L.gt_mask[i] >> (i - 1)) & 1) && !L.lt_mask[i];
+    L.clique = clq ? 1 : 0;
     c->lprog = L;
     c->local_ok = true;
 }
 
 enum class LocalPass { Wave, Group256, Group512, Group1024 };
 
-template <int N>
+template <int N, bool CLQ>
 static void launch_local(LocalPass pass, uint64_t blocks, uint32_t bw, hipStream_t s, const LocalArgs& a) {
     if constexpr (N > 2) {
         switch (pass) {
             case LocalPass::Group256:
-                hipLaunchKernelGGL((gm_local_block_kernel<N, kB1Threads, kB1RowWords, kB1U, kB1Buckets>), dim3(blocks),
+                hipLaunchKernelGGL((gm_local_block_kernel<N, CLQ, kB1Threads, kB1RowWords, kB1U, kB1Buckets>), dim3(blocks),
                                    dim3(kB1Threads), 0, s, a);
                 return;
             case LocalPass::Group512:
-                hipLaunchKernelGGL((gm_local_block_kernel<N, kB2Threads, kB2RowWords, kB2U, kB2Buckets>), dim3(blocks),
+                hipLaunchKernelGGL((gm_local_block_kernel<N, CLQ, kB2Threads, kB2RowWords, kB2U, kB2Buckets>), dim3(blocks),
                                    dim3(kB2Threads), 0, s, a);
                 return;
             case LocalPass::Group1024:
-                hipLaunchKernelGGL((gm_local_block_kernel<N, kB3Threads, kB3RowWords, kB3U, kB3Buckets>), dim3(blocks),
+                hipLaunchKernelGGL((gm_local_block_kernel<N, CLQ, kB3Threads, kB3RowWords, kB3U, kB3Buckets>), dim3(blocks),
                                    dim3(kB3Threads), 0, s, a);
                 return;
             default:
                 break;
         }
     }
-    hipLaunchKernelGGL(gm_local_kernel<N>, dim3(blocks), dim3(bw * kWave), 0, s, a);
+    hipLaunchKernelGGL((gm_local_kernel<N, CLQ>), dim3(blocks), dim3(bw * kWave), 0, s, a);
 }
 
 // Passes over the roots: rows in a wave's LDS; in a 256-, 512- and
@@ -1028,14 +1125,15 @@ int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st, bool bytes) {
     a.prog = c->lprog;
     a.ctr = ctr;
     auto launch = [&](LocalPass pass, uint64_t blocks) {
+        const bool clq = c->lprog.clique != 0;
         switch (c->lprog.n) {
-            case 2: launch_local<2>(pass, blocks, bw, c->stream, a); break;
-            case 3: launch_local<3>(pass, blocks, bw, c->stream, a); break;
-            case 4: launch_local<4>(pass, blocks, bw, c->stream, a); break;
-            case 5: launch_local<5>(pass, blocks, bw, c->stream, a); break;
-            case 6: launch_local<6>(pass, blocks, bw, c->stream, a); break;
-            case 7: launch_local<7>(pass, blocks, bw, c->stream, a); break;
-            default: launch_local<8>(pass, blocks, bw, c->stream, a); break;
+            case 2: launch_local<2, false>(pass, blocks, bw, c->stream, a); break;
+            case 3: clq ? launch_local<3, true>(pass, blocks, bw, c->stream, a) : launch_local<3, false>(pass, blocks, bw, c->stream, a); break;
+            case 4: clq ? launch_local<4, true>(pass, blocks, bw, c->stream, a) : launch_local<4, false>(pass, blocks, bw, c->stream, a); break;
+            case 5: clq ? launch_local<5, true>(pass, blocks, bw, c->stream, a) : launch_local<5, false>(pass, blocks, bw, c->stream, a); break;
+            case 6: clq ? launch_local<6, true>(pass, blocks, bw, c->stream, a) : launch_local<6, false>(pass, blocks, bw, c->stream, a); break;
+            case 7: clq ? launch_local<7, true>(pass, blocks, bw, c->stream, a) : launch_local<7, false>(pass, blocks, bw, c->stream, a); break;
+            default: clq ? launch_local<8, true>(pass, blocks, bw, c->stream, a) : launch_local<8, false>(pass, blocks, bw, c->stream, a); break;
         }
         return hipGetLastError();
     };

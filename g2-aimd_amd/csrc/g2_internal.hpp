@@ -65,6 +65,7 @@ struct LocalProgram {
     uint32_t gt_mask[G2_MAXQ];     // positions j >= 1 with M(u_i) > M(u_j)
     uint32_t lt_mask[G2_MAXQ];     // positions j >= 1 with M(u_i) < M(u_j)
     uint32_t ne_mask[G2_MAXQ];     // positions j >= 1 with M(u_i) != M(u_j)
+    uint32_t clique;               // every position i >= 2 adjacent to 1..i-1 and above i-1, no other masks
 };
 
 }  // namespace g2

@@ -112,6 +112,46 @@ def test_local_every_pass_takes_roots(g2, orc, gpu_exec):
         assert gpu_exec.run_gm(canonical=True)["count"] == ref
 
 
+def descending(g2, plan):
+    """The same clique plan with its symmetry breaking reversed: position
+    i >= 2 below position i-1 (LESS_THAN) instead of above it.  Still a
+    complete symmetry breaking, so the count is the same, but the plan is no
+    longer an ascending chain and takes the general walker (order ranges
+    over symmetric rows) instead of the clique kernels (rows of later members
+    only)."""
+    d = type(plan).from_buffer_copy(bytes(plan))
+    for i in range(2, d.n):
+        u = d.seq[i]
+        for t in range(d.cond_count[u]):
+            if d.rev[d.cond[u][t][1]] != 0:
+                d.cond[u][t][0] = 0
+    return d
+
+
+def test_clique_kernels_equal_general_walker(g2, orc, gpu_exec):
+    """Clique kernels and the general walker agree on totals and on partial
+    matches (nodes) in every pass, for 3- to 6-cliques."""
+    print("case: clique kernels vs general walker", flush=True)
+    g = gadget_graph(g2, 11)
+    gpu_exec.transfer(g)
+    for q in (0, 24, 25, 26):
+        plan = g2.plan_for_preset(q)
+        d = descending(g2, plan)
+        assert g2.plan_local_eligible(d)
+        gpu_exec.set_plan(plan)
+        a = gpu_exec.run_gm()
+        gpu_exec.set_plan(d)
+        b = gpu_exec.run_gm()
+        print(f"q={q}: clique {a['count']} nodes {a['stats']['nodes']}, general {b['count']} "
+              f"nodes {b['stats']['nodes']}, pass roots {a['stats']['local_pass_roots']}", flush=True)
+        assert a["stats"]["local_bitmap"] == b["stats"]["local_bitmap"] == 1
+        assert a["count"] == b["count"]
+        assert a["stats"]["nodes"] == b["stats"]["nodes"]
+        assert a["stats"]["local_pass_roots"] == b["stats"]["local_pass_roots"]
+        if q == 24:
+            assert a["count"] == orc.gm(g, orc.OrcPlan.from_buffer_copy(bytes(plan)))["count"]
+
+
 @pytest.mark.parametrize("q,k,root_degree", [(0, 3, False), (24, 4, False), (25, 5, False), (24, 4, True),
                                              (25, 5, True)])
 def test_local_view_bins(g2, orc, gpu_exec, q, k, root_degree):
