@@ -106,6 +106,23 @@ __device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ col
     return lo;
 }
 
+// lower bounds of x and of y in one sorted range, searched in lockstep so
+// that the two loads of every step are in flight together
+__device__ __forceinline__ void lower_bound2(const uint32_t* __restrict__ cols, uint64_t lo, uint64_t hi, uint32_t x,
+                                             uint32_t y, uint64_t* px, uint64_t* py) {
+    const uint64_t n = hi - lo;
+    uint64_t cx = 0, cy = 0, step = 1;  // counts of entries below x, below y
+    while (step * 2 <= n) step *= 2;
+    for (step = n ? step : 0; step; step >>= 1) {
+        const bool gx = cx + step <= n && cols[lo + cx + step - 1] < x;
+        const bool gy = cy + step <= n && cols[lo + cy + step - 1] < y;
+        cx += gx ? step : 0;
+        cy += gy ? step : 0;
+    }
+    *px = lo + cx;
+    *py = lo + cy;
+}
+
 // membership of val in the sorted row cols[lo, hi); an empty range is
 // "absent" (system/util.h:19-30 reads cols[lo] there instead)
 __device__ __forceinline__ bool contains(const uint32_t* __restrict__ cols, uint64_t lo, uint64_t hi, uint32_t val) {

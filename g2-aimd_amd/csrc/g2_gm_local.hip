@@ -89,6 +89,8 @@ struct LocalArgs {
     uint32_t* big_list;                 // deferred roots (null in the last pass)
     uint32_t defer_ctr;                 // LCTR_DEFER + 2 * pass: count of big_list, need next to it
     unsigned long long* ctr;
+    unsigned long long* tri;            // triangle features: count per rank
+    uint8_t* flag;                      // triangle features: flag per oriented entry
 };
 
 // u64 words between consecutive local rows: ceil(m/64) rounded up to odd, so
@@ -154,11 +156,15 @@ __device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint3
 // (looked up in N+(U_i), global memory).
 // Each lane keeps kItemsW items in flight; U is searched by a fixed number
 // of halving steps so the items' loads overlap.  rows and U are LDS or
-// global memory (fixed per call site after inlining).  CLQ (clique plans)
-// sets only the later member's bit: row i then holds the members above i.
-template <bool CLQ>
+// global memory (fixed per call site after inlining).  MODE 0 sets both
+// bits of a found pair; MODE 1 (clique plans) only the later member's, so
+// row i holds the members above i; MODE 2 sets no rows but counts the
+// triangle (root, U_i, U_j): one count for each member in a.tri, one for the
+// root in hits, and the flags of its three oriented entries (root row from
+// entry xlo on).
+template <int MODE>
 __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* U, uint64_t* rows, uint32_t m,
-                                            uint32_t W, uint32_t a0) {
+                                            uint32_t W, uint32_t a0, uint64_t xlo, uint32_t& hits) {
     const uint32_t lane = lane_id();
     const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
     uint32_t st = 0;  // (row offsets fit 32 bits: this path runs with fewer than 2^32 row entries)
@@ -172,8 +178,7 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
         // [U[ia+1], U[m-1]] and the cheaper side chosen again
         bool rev = rem * (1 + bitlen(e1 - e0)) < e1 - e0;
         if (!rev && e1 - e0 > kWave) {
-            e0 = lower_bound(a.ocols, e0, e1, U[ia + 1]);
-            e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
+            lower_bound2(a.ocols, e0, e1, U[ia + 1], U[m - 1] + 1, &e0, &e1);
             rev = rem * (1 + bitlen(e1 - e0)) < e1 - e0;
         }
         const uint64_t n = e1 - e0;
@@ -270,9 +275,18 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
                 hit = ib < m && U[ib] == w[q];
             }
             if (hit) {
-                atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
-                if constexpr (!CLQ)
-                    atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                if constexpr (MODE == 2) {
+                    hits++;
+                    atomicAdd(&a.tri[U[ia]], 1ull);
+                    atomicAdd(&a.tri[U[ib]], 1ull);
+                    a.flag[xlo + ia] = 1;
+                    a.flag[xlo + ib] = 1;
+                    a.flag[(cj[q] & kRev) ? (uint64_t)sj[q] + rpos[q] - 1 : (uint64_t)sj[q] + item[q]] = 1;
+                } else {
+                    atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+                    if constexpr (MODE == 0)
+                        atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                }
             }
         }
     }
@@ -377,7 +391,8 @@ __device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint
         for (uint32_t i = lane; i < m; i += kWave) Ucopy[i] = a.ocols[lo + i];
     wave_sync();
     for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) {
-        build_chunk<CLQ>(a, U, rows, m, W, a0);
+        uint32_t unused = 0;
+        build_chunk<CLQ ? 1 : 0>(a, U, rows, m, W, a0, 0, unused);
         wave_sync();  // (the chunk's atomics complete before the walk reads rows)
     }
     if constexpr (CLQ) {
@@ -470,7 +485,7 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
 // share the position-2 candidates of its r1, queued 64 at a time in LDS, and
 // each lane walks the rest of its (r1, r2) subtree.
 template <int N, bool CLQ, int BT, int ROWW, int BU, int NB>
-__global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
+__global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
     constexpr int kWaves = BT / kWave;
     constexpr int kMembersPerThread = (BU + BT - 1) / BT;
     __shared__ uint64_t rows[ROWW];
@@ -515,8 +530,13 @@ __global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
         __syncthreads();
         // value buckets over [U_0, U_{m-1}]: a lookup searches one bucket
         const uint32_t ulo = U[0];
-        const uint64_t bmul = ((uint64_t)NB << 32) / ((uint64_t)U[m - 1] - ulo + 1);
-        auto bucket = [&](uint32_t v) { return (uint32_t)(((uint64_t)(v - ulo) * bmul) >> 32); };
+        // bucket(v) = floor((v - ulo) * NB / range) by one 32-bit high multiply
+        // (the identity when U spans at most NB ranks); monotone, and below NB
+        // on [ulo, U_{m-1}]
+        const uint32_t range = U[m - 1] - ulo + 1;
+        const bool ident = range <= (uint32_t)NB;
+        const uint32_t bmul = ident ? 0u : (uint32_t)(((uint64_t)NB << 32) / range);
+        auto bucket = [&](uint32_t v) { return ident ? v - ulo : __umulhi(v - ulo, bmul); };
         for (uint32_t i = tid; i <= m; i += blockDim.x) {  // buckets (bucket(U_{i-1}), bucket(U_i)] start at i
             const uint32_t bi = i < m ? bucket(U[i]) : NB;
             for (uint32_t b = i == 0 ? 0 : bucket(U[i - 1]) + 1; b <= bi; b++) ubkt[b] = (uint16_t)i;
@@ -541,10 +561,7 @@ __global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
             if (i + 1 < m) {
                 const uint32_t u = U[i];
                 uint64_t e0 = a.orp[u], e1 = a.orp[u + 1];
-                if (e1 - e0 > kWave) {
-                    e0 = lower_bound(a.ocols, e0, e1, U[i + 1]);
-                    e1 = lower_bound(a.ocols, e0, e1, U[m - 1] + 1);
-                }
+                if (e1 - e0 > kWave) lower_bound2(a.ocols, e0, e1, U[i + 1], U[m - 1] + 1, &e0, &e1);
                 const uint64_t n = e1 - e0, rem = m - 1 - i;
                 const bool rev = n > (1u << 20) || rem * (1 + bitlen(n)) < n;
                 mstart[i] = (uint32_t)e0;
@@ -614,11 +631,17 @@ __global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
                     }
                 }
             }
+            // w: the entry to look up in U, or for a reversed member the later
+            // member to look up in the member's row (rlen > 0: that row's length)
+            uint32_t rst[kItems], rlen[kItems], rmax = 0;
 #pragma unroll
             for (int q = 0; q < kItems; q++) {
-                const uint32_t code = mcode[own[q]];
-                const uint32_t item = f[q] - mpref[own[q]];
-                w[q] = (f[q] < T && !(code & kRev)) ? a.ocols[(uint64_t)mstart[own[q]] + item] : ulo;
+                const uint32_t o = own[q], code = mcode[o], item = f[q] - mpref[o];
+                const bool valid = f[q] < T, rev = valid && (code & kRev);
+                rst[q] = mstart[o];
+                rlen[q] = rev ? (code & ~kRev) : 0;
+                rmax = max(rmax, rlen[q]);
+                w[q] = !valid ? ulo : rev ? U[o + 1 + item] : a.ocols[(uint64_t)rst[q] + item];
             }
             uint32_t lim[kItems];
 #pragma unroll
@@ -634,23 +657,47 @@ __global__ __launch_bounds__(BT) void gm_local_block_kernel(LocalArgs a) {
                     if (c <= lim[q] && U[c - 1] < w[q]) pos[q] = c;
                 }
             }
+            // reversed members: rpos = |{entries <= w}| in the member's row, the
+            // entry at rpos-1 kept; the items' searches interleave step by step
+            // so their global loads overlap
+            uint32_t rpos[kItems], last[kItems];
+#pragma unroll
+            for (int q = 0; q < kItems; q++) {
+                rpos[q] = 0;
+                last[q] = ~0u;
+            }
+            uint32_t rstep = 1;
+            while (rstep * 2 <= rmax) rstep *= 2;
+            for (uint32_t step = rmax ? rstep : 0; step >= 1; step >>= 1) {
+#pragma unroll
+                for (int q = 0; q < kItems; q++) {
+                    const uint32_t c = rpos[q] + step;
+                    if (c <= rlen[q]) {
+                        const uint32_t v = a.ocols[(uint64_t)rst[q] + c - 1];
+                        if (v <= w[q]) {
+                            rpos[q] = c;
+                            last[q] = v;
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int q = 0; q < kItems; q++) {
                 if (f[q] >= T) continue;
-                const uint32_t ia = own[q], code = mcode[ia];
+                const uint32_t ia = own[q];
                 uint32_t ib;
                 bool hit;
-                if (code & kRev) {
+                if (rlen[q]) {
                     ib = ia + 1 + (f[q] - mpref[ia]);
-                    hit = contains(a.ocols, mstart[ia], (uint64_t)mstart[ia] + (code & ~kRev), U[ib]);
+                    hit = last[q] == w[q];
                 } else {
                     ib = pos[q];
                     hit = ib < m && U[ib] == w[q];
                 }
-                if (hit) {
-                    atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+                if (hit) {  // (row offsets within the LDS rows fit 32 bits)
+                    atomicOr((unsigned long long*)&rows[ia * W + (ib >> 6)], 1ull << (ib & 63));
                     if constexpr (!CLQ)
-                        atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                        atomicOr((unsigned long long*)&rows[ib * W + (ia >> 6)], 1ull << (ia & 63));
                 }
             }
         }
@@ -979,6 +1026,96 @@ static int build_oriented(g2_ctx* c, const uint64_t** orp, const uint32_t** ocol
     *rank = pi;
     c->orient_valid = true;
     return G2_OK;
+}
+
+// ---- triangle features (learned scheduler) ---------------------------------
+// Every triangle x < y < z (ranks) is found once, at its lowest rank x, as
+// the pair (y, z) of U(x) = N+(x) with z in N+(y): the local rows' work list
+// without the rows (build_chunk MODE 2).  Its three vertices get one count
+// each and its three oriented entries (x,y), (x,z), (y,z) their flags.
+__global__ __launch_bounds__(256) void tri_kernel(LocalArgs a) {
+    __shared__ uint32_t lU[kLocalBlockWaves][kLU];
+    const uint32_t lane = lane_id();
+    const uint32_t wid = threadIdx.x / kWave;
+    uint64_t t = 0, tend = 0;
+    while (true) {
+        if (t == tend) {
+            t = fetch_tasks(&a.ctr[LCTR_NEXT], kFetchRun);
+            tend = min<uint64_t>(t + kFetchRun, a.ntasks);
+        }
+        if (t >= a.ntasks) break;
+        const uint32_t x = uniform((uint32_t)t);
+        t++;
+        const uint64_t lo = uniform(a.orp[x]);
+        const uint32_t m = (uint32_t)(uniform(a.orp[x + 1]) - lo);
+        if (m < 2) continue;
+        const uint32_t* U = a.ocols + lo;
+        wave_sync();  // (the previous root's reads of lU are done)
+        if (m <= kLU) {
+            for (uint32_t i = lane; i < m; i += kWave) lU[wid][i] = U[i];
+            wave_sync();
+            U = lU[wid];
+        }
+        uint32_t hits = 0;
+        for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) build_chunk<2>(a, U, nullptr, m, 0, a0, lo, hits);
+        const uint64_t tot = wave_sum((uint64_t)hits);
+        if (lane == 0 && tot) atomicAdd(&a.tri[x], (unsigned long long)tot);
+    }
+}
+
+// per rank: oriented entries (either end) that lie on a triangle
+__global__ __launch_bounds__(256) void sup_kernel(const uint64_t* __restrict__ orp, const uint32_t* __restrict__ ocols,
+                                                  const uint8_t* __restrict__ flag, uint64_t nv, uint32_t* sup) {
+    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / kWave;
+    for (uint64_t x = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; x < nv; x += nw) {
+        uint32_t c = 0;
+        for (uint64_t e = orp[x] + lane_id(); e < orp[x + 1]; e += kWave) {
+            if (flag[e]) {
+                c++;
+                atomicAdd(&sup[ocols[e]], 1u);
+            }
+        }
+        c = (uint32_t)wave_sum((uint64_t)c);
+        if (lane_id() == 0 && c) atomicAdd(&sup[x], c);
+    }
+}
+
+int local_triangle_features(g2_ctx* c, unsigned long long* tri, uint32_t* sup, uint8_t* flag, uint64_t flag_bytes,
+                            const uint32_t** rank) {
+    if (c->nv + 1 > 0x7fffffffull || c->ne >= (1ull << 32))
+        return set_err(c, G2_EINVAL, "oriented triangle features need fewer than 2^31-1 vertices and 2^32 entries");
+    if (flag_bytes < c->ne) return set_err(c, G2_EINVAL, "triangle flags: buffer too small");
+    int rc;
+    if ((rc = ensure(c, c->counters, LCTR_N * 8, "counters"))) return rc;
+    const uint64_t* orp = nullptr;
+    const uint32_t* ocols = nullptr;
+    if ((rc = build_oriented(c, &orp, &ocols, rank))) return rc;
+    hipError_t e;
+    const uint64_t nv = c->nv;
+    if ((e = hipMemsetAsync(c->counters.ptr, 0, LCTR_N * 8, c->stream))) return hip_err(c, e, "memset counters");
+    if ((e = hipMemsetAsync(tri, 0, std::max<uint64_t>(nv, 1) * 8, c->stream))) return hip_err(c, e, "memset tri");
+    if ((e = hipMemsetAsync(sup, 0, std::max<uint64_t>(nv, 1) * 4, c->stream))) return hip_err(c, e, "memset sup");
+    if ((e = hipMemsetAsync(flag, 0, std::max<uint64_t>(c->ne, 1), c->stream))) return hip_err(c, e, "memset flags");
+    LocalArgs a{};
+    a.orp = orp;
+    a.ocols = ocols;
+    a.ntasks = nv;
+    a.ctr = (unsigned long long*)c->counters.ptr;
+    a.tri = tri;
+    a.flag = flag;
+    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_GM_LOCAL) / kLocalBlockWaves);
+    hipLaunchKernelGGL(tri_kernel, dim3(blocks), dim3(kLocalBlockWaves * kWave), 0, c->stream, a);
+    const uint64_t wgrid = std::max<uint64_t>(1, (uint64_t)c->num_cus * 32 / 4);
+    hipLaunchKernelGGL(sup_kernel, dim3(wgrid), dim3(256), 0, c->stream, orp, ocols, (const uint8_t*)flag, nv, sup);
+    if ((e = hipGetLastError())) return hip_err(c, e, "triangle feature kernels");
+    return G2_OK;
+}
+
+void orient_release(g2_ctx* c) {
+    if (c->orient.ptr) (void)hipFree(c->orient.ptr);
+    c->used -= c->orient.bytes;
+    c->orient = DevBuf{};
+    c->orient_valid = false;
 }
 
 // ---- plan analysis (host) ---------------------------------------------------

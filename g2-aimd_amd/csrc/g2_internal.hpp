@@ -151,6 +151,13 @@ int gm_program(const g2_plan* p, GmProgram* out, const char** why);
 // gm_kernel (no byte accounting) over `blocks` workgroups of block_waves waves
 hipError_t gm_launch(const GmArgs& a, const GmProgram& prog, uint64_t blocks, uint32_t block_waves, hipStream_t s);
 void gm_local_compile(g2_ctx* c, const g2_plan* p);
+// triangle count per rank (tri, u64) and triangle-carrying oriented entries
+// per rank (sup, u32) of the uploaded view, through its oriented rows (built
+// and cached as for local matching); *rank = the rank of every vertex
+int local_triangle_features(g2_ctx* c, unsigned long long* tri, uint32_t* sup, uint8_t* flag, uint64_t flag_bytes,
+                            const uint32_t** rank);
+// drop the cached oriented rows (their memory returns to the budget)
+void orient_release(g2_ctx* c);
 int gm_local_run(g2_ctx* c, uint64_t* count, g2_stats* st, bool bytes);
 int gm_run_batch(g2_ctx* c, const g2_plan* plans, uint32_t nq, const g2_batch* cfg, uint64_t* counts,
                  uint64_t* qnodes, uint32_t* over, g2_stats* st);

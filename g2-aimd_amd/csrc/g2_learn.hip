@@ -5,8 +5,11 @@
 //     deg(v), 2-path count sum_{u in N(v)} (deg(u)-1), triangles t(v),
 //     neighbours sharing >= 1 common neighbour with v, and p0 = |{u > v}|
 //   (the size of the root's P set in the maximal-clique search tree);
-//   triangles are counted by streaming the shorter of N(v), N(u) and binary
-//   searching the other, one neighbour u at a time.
+//   triangles and triangle-carrying neighbours come from one pass over the
+//   view's oriented rows (local_triangle_features: each triangle found once,
+//   at its lowest (degree, id) rank) when every source's neighbours have
+//   their rows; otherwise each root streams the shorter of N(v), N(u) and
+//   binary searches the other, one neighbour u at a time.
 // * core_step_kernel: core numbers by the h-index iteration;
 //   inputs_kernel: the cost model's 8 inputs in fp32.
 // * mlp: 8 -> 64 -> 64 -> 1 with ReLU, BF16 weights and activations on the
@@ -36,6 +39,10 @@ struct FeatArgs {
     uint64_t ns;
     unsigned long long* next;
     int64_t* out;  // ns x G2_NFEAT_INT
+    // triangles and triangle-carrying neighbours per rank (null: counted here)
+    const unsigned long long* tri;
+    const uint32_t* sup;
+    const uint32_t* rank;
 };
 
 __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
@@ -59,7 +66,7 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
             two += (a.rp[u + 1] - a.rp[u]) - 1;
             p0 += u > v;
         }
-        for (uint64_t e = lo; e < hi; e++) {
+        for (uint64_t e = a.tri ? hi : lo; e < hi; e++) {
             const uint32_t u = a.cols[e];
             const uint64_t ulo = a.rp[u], uhi = a.rp[u + 1];
             uint64_t c = 0;
@@ -78,8 +85,8 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
             int64_t* o = a.out + ti * G2_NFEAT_INT;
             o[0] = (int64_t)d;
             o[1] = (int64_t)two;
-            o[2] = (int64_t)(tri2 / 2);
-            o[3] = (int64_t)shared;
+            o[2] = a.tri ? (int64_t)a.tri[a.rank[v]] : (int64_t)(tri2 / 2);
+            o[3] = a.tri ? (int64_t)a.sup[a.rank[v]] : (int64_t)shared;
             o[4] = (int64_t)p0;
         }
     }
@@ -198,9 +205,42 @@ int features_run(g2_ctx* c, int64_t* out) {
     DevBuf& dout = out_buf.b;
     if ((rc = ensure(c, dout, std::max<uint64_t>(c->ns, 1) * G2_NFEAT_INT * 8, "features"))) return rc;
     hipError_t e;
+    // triangles through the oriented rows when every source's neighbours have
+    // their rows (whole graph; views materialised with hop >= 2) and the
+    // budget holds the rows; the rows are dropped again unless a matching
+    // run had built them
+    const unsigned long long* tri = nullptr;
+    const uint32_t* sup = nullptr;
+    const uint32_t* rank = nullptr;
+    const bool had_orient = c->orient_valid;
+    struct TriBuf {
+        g2_ctx* c;
+        DevBuf b;
+        bool drop;
+        ~TriBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+            if (drop) orient_release(c);
+        }
+    } tri_buf{c, {}, false};
+    const uint64_t nv1 = std::max<uint64_t>(c->nv, 1), ne1 = std::max<uint64_t>(c->ne, 1);
+    const bool closed = c->sources_all || c->unclosed_max_degree == 0;
+    if (closed && c->ns && c->nv + 1 <= 0x7fffffffull && c->ne < (1ull << 32)) {
+        auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+        const uint64_t o_sup = al(nv1 * 8), o_flag = o_sup + al(nv1 * 4);
+        if (ensure(c, tri_buf.b, o_flag + ne1, "triangle features") == G2_OK) {
+            char* base = (char*)tri_buf.b.ptr;
+            tri_buf.drop = !had_orient;
+            if (local_triangle_features(c, (unsigned long long*)base, (uint32_t*)(base + o_sup),
+                                        (uint8_t*)(base + o_flag), ne1, &rank) == G2_OK) {
+                tri = (const unsigned long long*)base;
+                sup = (const uint32_t*)(base + o_sup);
+            }
+        }
+    }
     if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
     FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
-               (unsigned long long*)c->counters.ptr, (int64_t*)dout.ptr};
+               (unsigned long long*)c->counters.ptr, (int64_t*)dout.ptr, tri, sup, rank};
     const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
     hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
     if ((e = hipGetLastError())) return hip_err(c, e, "feature launch");

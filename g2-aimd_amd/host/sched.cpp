@@ -123,14 +123,27 @@ int g2h_learned_schedule(const float* pred, const int64_t* later, uint64_t ns, u
     if (ns >= 0xffffffffull) { g2h::set_error("at most 2^32-1 roots"); return -1; }
     if (!max_split) max_split = 4096;
     std::vector<double> cost(ns);
+    std::vector<uint64_t> a(ns), b(ns);  // ~bits(p) << 32 | index
     double sum = 0;
     for (uint64_t i = 0; i < ns; i++) {
-        const double p = std::isfinite(pred[i]) ? std::min(std::max((double)pred[i], 0.0), 60.0) : 0.0;
-        cost[i] = std::expm1(p);
+        const float p = std::isfinite(pred[i]) ? std::min(std::max(pred[i], 0.0f), 60.0f) : 0.0f;
+        cost[i] = std::expm1((double)p);
         sum += cost[i];
+        uint32_t bits;  // a non-negative float's bits order like its value
+        std::memcpy(&bits, &p, 4);
+        a[i] = (uint64_t)~bits << 32 | i;
     }
-    std::iota(order, order + ns, 0u);
-    std::stable_sort(order, order + ns, [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    // longest predicted first (expm1 is increasing, so ~bits(p) ascending is
+    // cost descending), ties in source order: a stable LSD radix sort on the
+    // high 32 bits, 8 per pass, of keys that start in index order
+    for (int shift = 32; shift < 64; shift += 8) {
+        uint64_t cnt[257] = {0};
+        for (uint64_t i = 0; i < ns; i++) cnt[((a[i] >> shift) & 255) + 1]++;
+        for (int k = 0; k < 256; k++) cnt[k + 1] += cnt[k];
+        for (uint64_t i = 0; i < ns; i++) b[cnt[(a[i] >> shift) & 255]++] = a[i];
+        a.swap(b);
+    }
+    for (uint64_t i = 0; i < ns; i++) order[i] = (uint32_t)a[i];
     const double share = std::max(sum / (double)std::max<uint64_t>(waves, 1), 1.0);
     for (uint64_t i = 0; i < ns; i++) {
         double k = std::ceil(cost[i] / share);

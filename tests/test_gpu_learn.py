@@ -37,6 +37,43 @@ def test_features(g2, gpu_exec):
         assert np.array_equal(f[:, 4], p0)
 
 
+def features_ref(g):
+    """deg, 2-paths, triangles, triangle-carrying neighbours, later neighbours
+    of every vertex, from scipy's sparse products."""
+    import scipy.sparse as sp
+    n = g.nv
+    src = np.repeat(np.arange(n), np.diff(g.row_ptrs).astype(np.int64))
+    A = sp.csr_matrix((np.ones(g.ne, np.int64), (src, g.cols.astype(np.int64))), shape=(n, n))
+    deg = np.asarray(A.sum(1)).ravel()
+    T = (A @ A).multiply(A)          # T[v, u] = common neighbours of the edge (v, u)
+    tri = np.asarray(T.sum(1)).ravel() // 2
+    shared = np.asarray((T > 0).sum(1)).ravel()
+    two = A @ deg - deg
+    p0 = np.asarray((sp.triu(A, 1) > 0).sum(1)).ravel()
+    return np.stack([deg, two, tri, shared, p0], axis=1)
+
+
+def test_features_oriented_and_views(g2, gpu_exec):
+    """Triangles through the oriented rows: R-MAT 13 whole graph; sources of
+    hop-2 views (every neighbour has its row) get their whole-graph
+    features; a hop-1 view (neighbours without rows) takes the per-neighbour
+    count and still gives exact degrees and later-neighbour counts."""
+    print("case: features, oriented rows and views", flush=True)
+    g = g2.gen_rmat(13, 16, 0.57, 0.19, 0.19, 0x6732A1D0C0000004).preprocess()
+    ref = features_ref(g)
+    gpu_exec.transfer(g)
+    assert np.array_equal(gpu_exec.features(), ref)
+    bins = np.arange(g.nv) % 3
+    for b in range(3):
+        src = np.nonzero(bins == b)[0].astype(np.uint32)
+        gpu_exec.transfer(g2.view_materialize(g, src, 2), sources=src)
+        assert np.array_equal(gpu_exec.features(), ref[src])
+    src = np.nonzero(bins == 0)[0].astype(np.uint32)
+    gpu_exec.transfer(g2.view_materialize(g, src, 1), sources=src)
+    f = gpu_exec.features()
+    assert np.array_equal(f[:, [0, 4]], ref[src][:, [0, 4]])
+
+
 def test_mlp_forward_and_grad(dev):
     print("case: mlp forward/grad", flush=True)
     rng = np.random.default_rng(3)

@@ -8,6 +8,7 @@ totals must equal the oracle's, since every count is a sum over roots."""
 import json
 import os
 import subprocess
+import threading
 
 import numpy as np
 import pytest
@@ -20,11 +21,24 @@ SEED = 0x6732A1D0C0000011
 
 
 def run_cli(name, args, out_json, timeout=90):
+    """Run a command line, echoing its output as it comes (long runs keep
+    printing), killed after `timeout` seconds."""
     cmd = [os.path.join(BIN, name)] + args + ["-json", str(out_json)]
     print("case:", " ".join(cmd), flush=True)
-    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout)
-    log = p.stdout.decode(errors="replace")
-    assert p.returncode == 0, log[-3000:]
+    lines = []
+    with subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                          errors="replace") as p:
+        timer = threading.Timer(timeout, p.kill)
+        timer.start()
+        try:
+            for line in p.stdout:
+                lines.append(line)
+                print("  |", line, end="", flush=True)
+            rc = p.wait()
+        finally:
+            timer.cancel()
+    log = "".join(lines)
+    assert rc == 0, f"rc={rc}\n" + log[-3000:]
     with open(out_json) as f:
         return json.load(f), log
 
@@ -78,7 +92,7 @@ def test_bk_learned_social_pipeline(g2, tmp_path):
     r, log = run_cli("bk", ["-gen", "social:1000000", "-h", "2", "-pn", "4", "-cn", "4", "-qs", "4", "-ss", "1",
                             "-sched", "both", "-train", str(model),
                             "-train-gen", "social:500000:0x6732A1D0C0000021,social:500000:0x6732A1D0C0000022",
-                            "-epochs", "10"], tmp_path / "social.json", timeout=400)
+                            "-epochs", "10", "-progress", "1"], tmp_path / "social.json", timeout=400)
     assert r["vertices"] == fx["nv"] and 2 * r["edges"] == fx["ne"]
     assert r["total"] == fx["max_cliques"]
     assert {int(k): v for k, v in r["hist"].items()} == {int(k): v for k, v in fx["hist"].items()}
@@ -90,10 +104,12 @@ def test_bk_learned_social_pipeline(g2, tmp_path):
     print(f"plain {r['plain']['seconds']:.3f}s kernel {r['plain']['kernel_ms']:.1f} ms idle "
           f"{r['plain']['wave_idle_frac']:.3f}; learned {r['learned']['seconds']:.3f}s kernel "
           f"{r['learned']['kernel_ms']:.1f} ms idle {r['learned']['wave_idle_frac']:.3f}", flush=True)
-    # the model file is reusable
-    r2, _ = run_cli("bk", ["-gen", "social:200000:0x6732A1D0C0000023", "-h", "2", "-pn", "2", "-cn", "2",
+    # the model file is reusable (on a training graph: other small seeds can
+    # hold a dense community with 1e9+ maximal cliques)
+    r2, _ = run_cli("bk", ["-gen", "social:500000:0x6732A1D0C0000021", "-h", "2", "-pn", "2", "-cn", "2",
                            "-qs", "2", "-sched", "both", "-model", str(model)], tmp_path / "again.json", timeout=200)
     assert r2["plain"]["total"] == r2["learned"]["total"] > 0
+    assert r2["plain"]["hist"] == r2["learned"]["hist"]
 
 
 def test_bk_aimd_retry(orc, g2, tmp_path):
