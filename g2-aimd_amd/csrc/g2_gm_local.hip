@@ -69,9 +69,9 @@ constexpr uint32_t kLRowWords = 448;    // per-wave LDS rows (u64 words)
 constexpr uint32_t kLU = 256;           // per-wave LDS copy of U (u32)
 // workgroup passes: threads, LDS row words, |U| limit, value buckets, and
 // workgroups per CU (the LDS each takes allows no more)
-constexpr int kB1Threads = 256, kB1RowWords = 3136, kB1U = 448, kB1Buckets = 512, kB1PerCu = 4;     // 25 KiB rows: |U| 448 at stride 7
-constexpr int kB2Threads = 512, kB2RowWords = 7700, kB2U = 700, kB2Buckets = 512, kB2PerCu = 2;      // 60 KiB rows: |U| 700 at stride 11
-constexpr int kB3Threads = 1024, kB3RowWords = 17664, kB3U = 1040, kB3Buckets = 512, kB3PerCu = 1;  // 138 KiB rows: |U| 1039 at stride 17
+constexpr int kB1Threads = 256, kB1RowWords = 3136, kB1U = 448, kB1Buckets = 2048, kB1PerCu = 4;     // 25 KiB rows: |U| 448 at stride 7
+constexpr int kB2Threads = 512, kB2RowWords = 7700, kB2U = 700, kB2Buckets = 2048, kB2PerCu = 2;      // 60 KiB rows: |U| 700 at stride 11
+constexpr int kB3Threads = 1024, kB3RowWords = 17664, kB3U = 1040, kB3Buckets = 2047, kB3PerCu = 1;  // 138 KiB rows: |U| 1039 at stride 17
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
@@ -233,9 +233,10 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
         }
         for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // stream: pos = |{U_j < w}|
 #pragma unroll
-            for (int q = 0; q < kItemsW; q++) {
+            for (int q = 0; q < kItemsW; q++) {  // (clamped read, no branch)
                 const uint32_t c = pos[q] + step;
-                if (!(cj[q] & kRev) && c <= m && U[c - 1] < w[q]) pos[q] = c;
+                const uint32_t v = U[min(c, m) - 1];
+                pos[q] = (!(cj[q] & kRev) && c <= m && v < w[q]) ? c : pos[q];
             }
         }
         // lookups: rpos = |{entries <= w}| in the member's row, with the entry
@@ -272,7 +273,8 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
                 hit = rpos[q] > 0 && last[q] == w[q];
             } else {
                 ib = pos[q];
-                hit = ib < m && U[ib] == w[q];
+                const uint32_t uv = U[min(ib, m - 1)];
+                hit = ib < m && uv == w[q];
             }
             if (hit) {
                 if constexpr (MODE == 2) {
@@ -493,11 +495,16 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
     __shared__ uint32_t mpref[BU];   // members' first work item
     __shared__ uint32_t mstart[BU];  // members' first entry in ocols
     __shared__ uint32_t mcode[BU];   // members' entry count | kRev
-    __shared__ uint32_t queue[kWaves][kWave];
     __shared__ uint32_t wsum[kWaves];
     __shared__ unsigned long long s_task;
     __shared__ uint32_t s_next, s_maxb;
-    __shared__ uint16_t ubkt[NB + 1];  // first index of U in each value bucket
+    // the build's value-bucket index (first index of U in each bucket) and the
+    // count's per-wave candidate queues share one buffer: a barrier separates
+    // the build from the count, and the next root's index from this count
+    constexpr int kQueueBytes = kWaves * kWave * 4, kBucketBytes = (NB + 1) * 2;
+    __shared__ __align__(16) uint8_t qb_raw[kQueueBytes > kBucketBytes ? kQueueBytes : kBucketBytes];
+    auto queue = reinterpret_cast<uint32_t (*)[kWave]>(qb_raw);
+    uint16_t* const ubkt = reinterpret_cast<uint16_t*>(qb_raw);
     const uint32_t lane = lane_id();
     const uint32_t wid = threadIdx.x / kWave;
     const uint32_t nwaves = blockDim.x / kWave;
@@ -614,6 +621,8 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
                 if (o0 + step < nm && mpref[o0 + step] <= f0) o0 += step;
                 if (o1 + step < nm && mpref[o1 + step] <= flast) o1 += step;
             }
+            o0 = __builtin_amdgcn_readfirstlane(o0);  // (wave-uniform: scalar from here)
+            o1 = __builtin_amdgcn_readfirstlane(o1);
             uint32_t hb_r = 1;
             while (hb_r * 2 <= o1 - o0) hb_r *= 2;
             uint32_t f[kItems], own[kItems], w[kItems], pos[kItems];

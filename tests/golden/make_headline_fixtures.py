@@ -33,10 +33,14 @@ RMAT_SEED = 0x6732A1D0C0000002
 SOCIAL_SEED = 0x6732A1D0C0000003
 
 
-def fingerprint(g):
-    """(V, E, sum over i of cols[i] * (i + 1) mod 2^64) of a CSR graph."""
-    w = np.arange(1, g.ne + 1, dtype=np.uint64)
-    chk = int(np.sum(g.cols.astype(np.uint64) * w, dtype=np.uint64))
+def fingerprint(g, chunk=1 << 26):
+    """(V, E, sum over i of cols[i] * (i + 1) mod 2^64) of a CSR graph, in
+    chunks (a 65M-vertex graph has 3.6e9 entries)."""
+    chk = 0
+    for a in range(0, g.ne, chunk):
+        b = min(a + chunk, g.ne)
+        w = np.arange(a + 1, b + 1, dtype=np.uint64)
+        chk = (chk + int(np.sum(g.cols[a:b].astype(np.uint64) * w, dtype=np.uint64))) % (1 << 64)
     return {"nv": int(g.nv), "ne": int(g.ne), "cols_checksum": chk}
 
 
@@ -76,11 +80,13 @@ def rmat_entry(g2, orc, scale, threads):
 
 def social_entry(g2, orc, n, threads):
     g = social_graph(g2, n)
+    fp = fingerprint(g)
+    print(f"social-{n}: {fp}", flush=True)
     t = time.time()
     r = orc.mce(g, threads=threads, hist_len=256)
     hist = {int(k): int(v) for k, v in enumerate(r["hist"]) if v}
     e = {"graph": f"social generator n={n}, seed {SOCIAL_SEED:#x}, preprocessed, degeneracy order",
-         **fingerprint(g), "max_cliques": r["total"], "hist": hist, "nodes": r["nodes"],
+         **fp, "max_cliques": r["total"], "hist": hist, "nodes": r["nodes"],
          "seconds": round(time.time() - t, 1)}
     print(f"social-{n}: {r['total']} maximal cliques in {e['seconds']}s", flush=True)
     return e
