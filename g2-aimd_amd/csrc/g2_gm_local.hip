@@ -77,6 +77,8 @@ constexpr int kItems = 8;               // work items per thread in flight in th
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
 constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
 constexpr uint32_t kFetchRun = 8;       // roots per counter fetch in the first pass (a sweep of 4..64 favoured 8)
+constexpr uint32_t kWB = 256;           // value buckets over U for a root in the wave's LDS (|U| <= 149: u8 indices)
+constexpr uint32_t kWBMin = 32;         // smaller U: a search of all of U is as short
 
 struct LocalArgs {
     const uint64_t* __restrict__ orp;   // oriented rows over ranks
@@ -162,9 +164,21 @@ __device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint3
 // triangle (root, U_i, U_j): one count for each member in a.tri, one for the
 // root in hits, and the flags of its three oriented entries (root row from
 // entry xlo on).
+// Value buckets over U (optional): bucket(v) = floor((v - ulo) * NB / range)
+// by one high multiply (the identity when U spans at most NB ranks); a
+// lookup of w searches U[bkt[b], bkt[b+1]) for b = bucket(w), in hb halving
+// steps (hb: the highest power of two <= the largest bucket).
+struct UBuckets {
+    const uint8_t* bkt;  // null: search all of U
+    uint32_t nb, ulo, bmul, hb;
+    bool ident;
+    __device__ uint32_t bucket(uint32_t v) const { return min(ident ? v - ulo : __umulhi(v - ulo, bmul), nb - 1); }
+};
+
 template <int MODE>
 __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* U, uint64_t* rows, uint32_t m,
-                                            uint32_t W, uint32_t a0, uint64_t xlo, uint32_t& hits) {
+                                            uint32_t W, uint32_t a0, uint64_t xlo, uint32_t& hits,
+                                            const UBuckets& B) {
     const uint32_t lane = lane_id();
     const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
     uint32_t st = 0;  // (row offsets fit 32 bits: this path runs with fewer than 2^32 row entries)
@@ -231,12 +245,25 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             }
             pos[q] = 0;
         }
-        for (uint32_t step = hb_u; step >= 1; step >>= 1) {  // stream: pos = |{U_j < w}|
+        uint32_t lim[kItemsW];
+        uint32_t hb = hb_u;
+#pragma unroll
+        for (int q = 0; q < kItemsW; q++) lim[q] = m;
+        if (B.bkt) {  // (an entry above U_{m-1} lands in the last bucket and misses)
+            hb = B.hb;
+#pragma unroll
+            for (int q = 0; q < kItemsW; q++) {
+                const uint32_t b = B.bucket(w[q]);
+                pos[q] = B.bkt[b];
+                lim[q] = B.bkt[b + 1];
+            }
+        }
+        for (uint32_t step = hb; step >= 1; step >>= 1) {  // stream: pos = |{U_j < w}|
 #pragma unroll
             for (int q = 0; q < kItemsW; q++) {  // (clamped read, no branch)
                 const uint32_t c = pos[q] + step;
                 const uint32_t v = U[min(c, m) - 1];
-                pos[q] = (!(cj[q] & kRev) && c <= m && v < w[q]) ? c : pos[q];
+                pos[q] = (!(cj[q] & kRev) && c <= lim[q] && v < w[q]) ? c : pos[q];
             }
         }
         // lookups: rpos = |{entries <= w}| in the member's row, with the entry
@@ -384,7 +411,7 @@ __device__ __forceinline__ void walk_from(const LocalProgram& P, uint32_t* M, ui
 // matches of the position-1 candidates lane, lane+64, ...
 template <int N, bool CLQ>
 __device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint32_t m, uint64_t* rows,
-                                           const uint32_t* U, uint32_t* Ucopy, uint64_t& lane_total,
+                                           const uint32_t* U, uint32_t* Ucopy, uint8_t* bkt, uint64_t& lane_total,
                                            uint64_t& nodes) {
     const uint32_t lane = lane_id();
     const uint32_t W = row_stride(m);
@@ -392,9 +419,28 @@ __device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint
     if (Ucopy)
         for (uint32_t i = lane; i < m; i += kWave) Ucopy[i] = a.ocols[lo + i];
     wave_sync();
+    UBuckets B{};
+    if (bkt && m > kWBMin) {  // value buckets over U (U in LDS, m <= 149)
+        const uint32_t range = U[m - 1] - U[0] + 1;
+        B.nb = kWB;
+        B.ulo = U[0];
+        B.ident = range <= kWB;
+        B.bmul = B.ident ? 0u : (uint32_t)(((uint64_t)kWB << 32) / range);
+        for (uint32_t i = lane; i <= m; i += kWave) {  // buckets (bucket(U_{i-1}), bucket(U_i)] start at i
+            const uint32_t bi = i < m ? B.bucket(U[i]) : kWB;
+            for (uint32_t b = i == 0 ? 0 : B.bucket(U[i - 1]) + 1; b <= bi; b++) bkt[b] = (uint8_t)i;
+        }
+        wave_sync();
+        uint32_t big = 0;
+        for (uint32_t b = lane; b < kWB; b += kWave) big = max<uint32_t>(big, bkt[b + 1] - bkt[b]);
+        big = (uint32_t)wave_max((uint64_t)big);
+        B.hb = 1;
+        while (B.hb * 2 <= big) B.hb *= 2;
+        B.bkt = bkt;
+    }
     for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) {
         uint32_t unused = 0;
-        build_chunk<CLQ ? 1 : 0>(a, U, rows, m, W, a0, 0, unused);
+        build_chunk<CLQ ? 1 : 0>(a, U, rows, m, W, a0, 0, unused, B);
         wave_sync();  // (the chunk's atomics complete before the walk reads rows)
     }
     if constexpr (CLQ) {
@@ -433,6 +479,7 @@ template <int N, bool CLQ>
 __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalArgs a) {
     __shared__ uint64_t lrows[kLocalBlockWaves][kLRowWords];
     __shared__ uint32_t lU[kLocalBlockWaves][kLU];
+    __shared__ uint8_t lB[kLocalBlockWaves][kWB + 4];
     const uint32_t lane = lane_id();
     const uint32_t wid = threadIdx.x / kWave;
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + wid;
@@ -461,9 +508,9 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
         } else {
             const uint64_t need = local_need(m);
             if (m <= kLU && need <= kLRowWords) {
-                local_root<N, CLQ>(a, lo, m, lrows[wid], lU[wid], lU[wid], lane_total, nodes);
+                local_root<N, CLQ>(a, lo, m, lrows[wid], lU[wid], lU[wid], lB[wid], lane_total, nodes);
             } else if (!a.big_list && need <= a.scratch_words) {
-                local_root<N, CLQ>(a, lo, m, ws, a.ocols + lo, nullptr, lane_total, nodes);
+                local_root<N, CLQ>(a, lo, m, ws, a.ocols + lo, nullptr, nullptr, lane_total, nodes);
             } else if (lane == 0) {
                 defer_root(a, x, need);
             }
@@ -1066,7 +1113,7 @@ __global__ __launch_bounds__(256) void tri_kernel(LocalArgs a) {
             U = lU[wid];
         }
         uint32_t hits = 0;
-        for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) build_chunk<2>(a, U, nullptr, m, 0, a0, lo, hits);
+        for (uint32_t a0 = 0; a0 + 1 < m; a0 += kWave) build_chunk<2>(a, U, nullptr, m, 0, a0, lo, hits, UBuckets{});
         const uint64_t tot = wave_sum((uint64_t)hits);
         if (lane == 0 && tot) atomicAdd(&a.tri[x], (unsigned long long)tot);
     }
