@@ -106,6 +106,13 @@ __device__ __forceinline__ uint64_t lower_bound(const uint32_t* __restrict__ col
     return lo;
 }
 
+// integer min / max (explicit template arguments to HIP's min<T> select its
+// floating-point template, which converts through f64)
+__host__ __device__ __forceinline__ uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ uint32_t umax32(uint32_t a, uint32_t b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
 // lower bounds of x and of y in one sorted range, searched in lockstep so
 // that the two loads of every step are in flight together
 __device__ __forceinline__ void lower_bound2(const uint32_t* __restrict__ cols, uint64_t lo, uint64_t hi, uint32_t x,

@@ -171,8 +171,11 @@ __device__ inline void cand_range(const LocalProgram& P, uint32_t i, const uint3
 struct UBuckets {
     const uint8_t* bkt;  // null: search all of U
     uint32_t nb, ulo, bmul, hb;
-    bool ident;
-    __device__ uint32_t bucket(uint32_t v) const { return min(ident ? v - ulo : __umulhi(v - ulo, bmul), nb - 1); }
+    uint32_t imask;      // ~0 when U spans at most nb ranks (then bmul = 0 and the bucket is v - ulo), else 0
+    __device__ uint32_t bucket(uint32_t v) const {
+        const uint32_t d = v - ulo;
+        return min(__umulhi(d, bmul) + (d & imask), nb - 1);
+    }
 };
 
 template <int MODE>
@@ -180,7 +183,7 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
                                             uint32_t W, uint32_t a0, uint64_t xlo, uint32_t& hits,
                                             const UBuckets& B) {
     const uint32_t lane = lane_id();
-    const uint32_t na = min<uint32_t>(kWave, m - 1 - a0);  // members with a later member
+    const uint32_t na = umin32(kWave, m - 1 - a0);  // members with a later member
     uint32_t st = 0;  // (row offsets fit 32 bits: this path runs with fewer than 2^32 row entries)
     uint32_t code = 0, len = 0;
     if (lane < na) {
@@ -424,15 +427,17 @@ __device__ __forceinline__ void local_root(const LocalArgs& a, uint64_t lo, uint
         const uint32_t range = U[m - 1] - U[0] + 1;
         B.nb = kWB;
         B.ulo = U[0];
-        B.ident = range <= kWB;
-        B.bmul = B.ident ? 0u : (uint32_t)(((uint64_t)kWB << 32) / range);
+        // (uniform values made opaque to the compiler: it would otherwise
+        // widen the high multiply to 64 bits)
+        B.imask = __builtin_amdgcn_readfirstlane(range <= kWB ? ~0u : 0u);
+        B.bmul = __builtin_amdgcn_readfirstlane(range <= kWB ? 0u : (uint32_t)(((uint64_t)kWB << 32) / range));
         for (uint32_t i = lane; i <= m; i += kWave) {  // buckets (bucket(U_{i-1}), bucket(U_i)] start at i
             const uint32_t bi = i < m ? B.bucket(U[i]) : kWB;
             for (uint32_t b = i == 0 ? 0 : B.bucket(U[i - 1]) + 1; b <= bi; b++) bkt[b] = (uint8_t)i;
         }
         wave_sync();
         uint32_t big = 0;
-        for (uint32_t b = lane; b < kWB; b += kWave) big = max<uint32_t>(big, bkt[b + 1] - bkt[b]);
+        for (uint32_t b = lane; b < kWB; b += kWave) big = umax32(big, bkt[b + 1] - bkt[b]);
         big = (uint32_t)wave_max((uint64_t)big);
         B.hb = 1;
         while (B.hb * 2 <= big) B.hb *= 2;
@@ -494,7 +499,7 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
     while (true) {
         if (t == tend) {
             t = fetch_tasks(&a.ctr[LCTR_NEXT], run);
-            tend = min<uint64_t>(t + run, a.ntasks);
+            tend = umin64(t + run, a.ntasks);
         }
         if (t >= a.ntasks) break;
         const uint32_t x = uniform(a.list ? a.list[t] : (uint32_t)t);
@@ -588,9 +593,15 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
         // (the identity when U spans at most NB ranks); monotone, and below NB
         // on [ulo, U_{m-1}]
         const uint32_t range = U[m - 1] - ulo + 1;
-        const bool ident = range <= (uint32_t)NB;
-        const uint32_t bmul = ident ? 0u : (uint32_t)(((uint64_t)NB << 32) / range);
-        auto bucket = [&](uint32_t v) { return ident ? v - ulo : __umulhi(v - ulo, bmul); };
+        // (uniform values made opaque to the compiler: it would otherwise
+        // widen the high multiply to 64 bits)
+        const uint32_t imask = __builtin_amdgcn_readfirstlane(range <= (uint32_t)NB ? ~0u : 0u);
+        const uint32_t bmul =
+            __builtin_amdgcn_readfirstlane(range <= (uint32_t)NB ? 0u : (uint32_t)(((uint64_t)NB << 32) / range));
+        auto bucket = [&](uint32_t v) {
+            const uint32_t d = v - ulo;
+            return __umulhi(d, bmul) + (d & imask);
+        };
         for (uint32_t i = tid; i <= m; i += blockDim.x) {  // buckets (bucket(U_{i-1}), bucket(U_i)] start at i
             const uint32_t bi = i < m ? bucket(U[i]) : NB;
             for (uint32_t b = i == 0 ? 0 : bucket(U[i - 1]) + 1; b <= bi; b++) ubkt[b] = (uint16_t)i;
@@ -598,7 +609,7 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
         __syncthreads();
         {
             uint32_t big = 0;
-            for (uint32_t b = tid; b < NB; b += blockDim.x) big = max<uint32_t>(big, ubkt[b + 1] - ubkt[b]);
+            for (uint32_t b = tid; b < NB; b += blockDim.x) big = umax32(big, ubkt[b + 1] - ubkt[b]);
             big = (uint32_t)wave_max((uint64_t)big);
             if (lane == 0 && big) atomicMax(&s_maxb, big);
         }
@@ -702,7 +713,7 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
             uint32_t lim[kItems];
 #pragma unroll
             for (int q = 0; q < kItems; q++) {  // (an entry above U_{m-1} lands in the last bucket and misses)
-                const uint32_t b = min<uint32_t>(bucket(w[q]), NB - 1);
+                const uint32_t b = umin32(bucket(w[q]), NB - 1);
                 pos[q] = ubkt[b];
                 lim[q] = ubkt[b + 1];
             }
@@ -955,7 +966,7 @@ __global__ __launch_bounds__(256) void ofill_kernel(const uint64_t* __restrict__
             uint64_t r = 0;
             for (uint64_t c0 = 0; c0 < m; c0 += kWave) {
                 const uint32_t vc = c0 + lane < m ? src[c0 + lane] : 0xffffffffu;
-                const uint32_t nc = (uint32_t)min<uint64_t>(kWave, m - c0);
+                const uint32_t nc = (uint32_t)umin64(kWave, m - c0);
                 for (uint32_t q = 0; q < nc; q++) {
                     const uint32_t o = read_lane(vc, q);
                     r += o < my || (o == my && c0 + q < i);
@@ -1000,7 +1011,7 @@ __global__ __launch_bounds__(256) void local_bytes_kernel(const uint64_t* __rest
             for (uint64_t i = lane; i + 1 < m; i += kWave) {
                 const uint32_t u = ocols[lo + i];
                 const uint64_t d = orp[u + 1] - orp[u], later = m - 1 - i;
-                b += 16 + 4 * min<uint64_t>(d, later * (1 + bitlen(d)));
+                b += 16 + 4 * umin64(d, later * (1 + bitlen(d)));
             }
             b = wave_sum(b);
             b += 4 * m;
@@ -1097,7 +1108,7 @@ __global__ __launch_bounds__(256) void tri_kernel(LocalArgs a) {
     while (true) {
         if (t == tend) {
             t = fetch_tasks(&a.ctr[LCTR_NEXT], kFetchRun);
-            tend = min<uint64_t>(t + kFetchRun, a.ntasks);
+            tend = umin64(t + kFetchRun, a.ntasks);
         }
         if (t >= a.ntasks) break;
         const uint32_t x = uniform((uint32_t)t);

@@ -24,6 +24,8 @@
 #include <random>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "g2_device.hpp"
 #include "g2_internal.hpp"
 
@@ -54,7 +56,7 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
     while (true) {
         if (t == tend) {
             t = fetch_tasks(a.next, kRun);
-            tend = min<uint64_t>(t + kRun, a.ns);
+            tend = umin64(t + kRun, a.ns);
         }
         if (t >= a.ns) break;
         const uint64_t ti = t++;
@@ -111,7 +113,7 @@ __device__ inline uint32_t count_at_least(const uint32_t* __restrict__ cols, uin
 __global__ __launch_bounds__(256) void core_init_kernel(const uint64_t* __restrict__ rp, uint64_t nv, uint32_t* h,
                                                         uint8_t* act) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
-        h[v] = (uint32_t)min<uint64_t>(rp[v + 1] - rp[v], 0xffffffffull);
+        h[v] = (uint32_t)umin64(rp[v + 1] - rp[v], 0xffffffffull);
         act[v] = 1;
     }
 }
@@ -190,20 +192,57 @@ __global__ __launch_bounds__(256) void inputs_kernel(const int64_t* __restrict__
     }
 }
 
-int features_run(g2_ctx* c, int64_t* out) {
+// ---- the learned schedule on the device ------------------------------------
+// standardised inputs, as g2h_norm_apply: (float)(((double)x - mu) / sd)
+__global__ __launch_bounds__(256) void norm_kernel(float* x, uint64_t n, const float* __restrict__ musd) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * 8; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = (uint32_t)(i & 7);
+        x[i] = (float)(((double)x[i] - (double)musd[k]) / (double)musd[8 + k]);
+    }
+}
+
+// clamp(pred, 0, 60) (NaN: 0), the predicted cost of a root
+__device__ inline float sched_p(float y) { return isfinite(y) ? fminf(fmaxf(y, 0.0f), 60.0f) : 0.0f; }
+
+// order keys ~bits(p) (ascending = longest predicted first) with the source
+// index as value, and per-block sums of the costs expm1(p) in a fixed order
+__global__ __launch_bounds__(256) void sched_keys_kernel(const float* __restrict__ pred, uint64_t n, uint32_t* keys,
+                                                         uint32_t* idx, double* partial) {
+    __shared__ double red[256];
+    double acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float p = sched_p(pred[i]);
+        keys[i] = ~__float_as_uint(p);
+        idx[i] = (uint32_t)i;
+        acc += expm1((double)p);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// parts per root: ceil(cost / share), at most its later neighbours (the
+// first-level children it can have) and max_split, at least 1
+__global__ __launch_bounds__(256) void sched_split_kernel(const float* __restrict__ pred, const int64_t* __restrict__ feat,
+                                                          uint64_t n, double share, uint32_t max_split, uint32_t* split) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int64_t later = feat[i * G2_NFEAT_INT + 4];
+        double k = ceil(expm1((double)sched_p(pred[i])) / share);
+        k = fmin(k, (double)(later > 1 ? later : 1));
+        k = fmin(fmax(k, 1.0), (double)max_split);
+        split[i] = (uint32_t)k;
+    }
+}
+
+// features of every uploaded source into dfeat (device, ns x G2_NFEAT_INT);
+// returns after the device has finished (its triangle buffers are freed)
+static int features_dev(g2_ctx* c, int64_t* dfeat) {
     int rc;
     if ((rc = ensure(c, c->counters, 8, "counters"))) return rc;
-    // the output buffer lives for this call only, on every return path
-    struct CallBuf {
-        g2_ctx* c;
-        DevBuf b;
-        ~CallBuf() {
-            if (b.ptr) (void)hipFree(b.ptr);
-            c->used -= b.bytes;
-        }
-    } out_buf{c, {}};
-    DevBuf& dout = out_buf.b;
-    if ((rc = ensure(c, dout, std::max<uint64_t>(c->ns, 1) * G2_NFEAT_INT * 8, "features"))) return rc;
     hipError_t e;
     // triangles through the oriented rows when every source's neighbours have
     // their rows (whole graph; views materialised with hop >= 2) and the
@@ -240,13 +279,31 @@ int features_run(g2_ctx* c, int64_t* out) {
     }
     if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
     FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
-               (unsigned long long*)c->counters.ptr, (int64_t*)dout.ptr, tri, sup, rank};
+               (unsigned long long*)c->counters.ptr, dfeat, tri, sup, rank};
     const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
     hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
     if ((e = hipGetLastError())) return hip_err(c, e, "feature launch");
-    if (c->ns && (e = hipMemcpyAsync(out, dout.ptr, c->ns * G2_NFEAT_INT * 8, hipMemcpyDeviceToHost, c->stream)))
-        return hip_err(c, e, "D2H features");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "feature sync");
+    return G2_OK;
+}
+
+int features_run(g2_ctx* c, int64_t* out) {
+    // the output buffer lives for this call only, on every return path
+    struct CallBuf {
+        g2_ctx* c;
+        DevBuf b;
+        ~CallBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+        }
+    } out_buf{c, {}};
+    DevBuf& dout = out_buf.b;
+    int rc;
+    if ((rc = ensure(c, dout, std::max<uint64_t>(c->ns, 1) * G2_NFEAT_INT * 8, "features"))) return rc;
+    if ((rc = features_dev(c, (int64_t*)dout.ptr))) return rc;
+    hipError_t e;
+    if (c->ns && (e = hipMemcpy(out, dout.ptr, c->ns * G2_NFEAT_INT * 8, hipMemcpyDeviceToHost)))
+        return hip_err(c, e, "D2H features");
     return G2_OK;
 }
 
@@ -678,11 +735,7 @@ int g2_root_inputs(g2_ctx* c, const uint32_t* core, float* out) {
     } else if ((rc = core_numbers_run(c, dcore, (uint8_t*)(base + o_act), (uint8_t*)(base + o_act + al(nv))))) {
         return rc;
     }
-    if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
-    FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
-               (unsigned long long*)c->counters.ptr, feat};
-    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
-    hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
+    if ((rc = features_dev(c, feat))) return rc;
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((c->ns + 255) / 256, 8192));
     hipLaunchKernelGGL(inputs_kernel, dim3(grid), dim3(256), 0, c->stream, feat, (const uint32_t*)c->src.ptr, dcore,
                        c->ns, c->nv, x);
@@ -813,6 +866,97 @@ int g2_mlp_train(g2_mlp* M, const float* x, const float* y, uint64_t n, uint32_t
         last = sse / n;
     }
     if (loss_out) *loss_out = last;
+    return G2_OK;
+}
+
+// The learned schedule of the uploaded view without leaving the device:
+// features, the 8 inputs, their standardisation, the cost model, then the
+// order (stable radix sort of the cost keys) and the parts.  The rule is
+// g2h_learned_schedule's (share = max(sum of costs / waves, 1)).
+int g2_learned_schedule(g2_ctx* c, g2_mlp* M, const float* mu, const float* sd, const uint32_t* core, uint64_t waves,
+                        uint32_t max_split, uint32_t* order, uint32_t* split, float* pred) {
+    if (!c || !M || !mu || !sd) return G2_EINVAL;
+    if (!c->view_ok) return set_err(c, G2_ENOVIEW, "no view uploaded");
+    if (M->device != c->device) return set_err(c, G2_EINVAL, "the cost model lives on another device");
+    const uint64_t ns = c->ns, nv = std::max<uint64_t>(c->nv, 1);
+    if (ns && (!order || !split)) return set_err(c, G2_EINVAL, "null order or split");
+    if (ns >= 0xffffffffull) return set_err(c, G2_EINVAL, "at most 2^32-1 roots");
+    if (!ns) return G2_OK;
+    if (!max_split) max_split = 4096;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_err(c, e, "hipSetDevice");
+    if (mlp_ensure(M, ns)) return set_err(c, G2_EHIP, "cost model buffers");
+    int rc;
+    if ((rc = ensure(c, c->counters, 8, "counters"))) return rc;
+    struct CallBuf {
+        g2_ctx* c;
+        DevBuf b;
+        ~CallBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+        }
+    } buf{c, {}};
+    constexpr uint32_t kKeyBlocks = 1024;
+    const uint64_t kb = std::min<uint64_t>(kKeyBlocks, (ns + 255) / 256);
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    size_t t_sort = 0;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)ns, 0, 32, c->stream)))
+        return hip_err(c, e, "sort size");
+    const uint64_t o_core = al(ns * G2_NFEAT_INT * 8), o_act = o_core + al(nv * 4), o_musd = o_act + 2 * al(nv),
+                   o_keys = o_musd + 256, o_keys2 = o_keys + al(ns * 4), o_idx = o_keys2 + al(ns * 4),
+                   o_order = o_idx + al(ns * 4), o_split = o_order + al(ns * 4), o_part = o_split + al(ns * 4),
+                   o_temp = o_part + al(kb * 8), total = o_temp + al(t_sort);
+    if ((rc = ensure(c, buf.b, total, "learned schedule"))) return rc;
+    char* base = (char*)buf.b.ptr;
+    auto* feat = (int64_t*)base;
+    auto* dcore = (uint32_t*)(base + o_core);
+    auto* musd = (float*)(base + o_musd);
+    auto* keys = (uint32_t*)(base + o_keys);
+    auto* keys2 = (uint32_t*)(base + o_keys2);
+    auto* idx = (uint32_t*)(base + o_idx);
+    auto* dorder = (uint32_t*)(base + o_order);
+    auto* dsplit = (uint32_t*)(base + o_split);
+    auto* part = (double*)(base + o_part);
+    if (core) {
+        if ((e = hipMemcpyAsync(dcore, core, c->nv * 4, hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D core");
+    } else if ((rc = core_numbers_run(c, dcore, (uint8_t*)(base + o_act), (uint8_t*)(base + o_act + al(nv))))) {
+        return rc;
+    }
+    float h_musd[16];
+    for (int k = 0; k < 8; k++) {
+        h_musd[k] = mu[k];
+        h_musd[8 + k] = sd[k];
+    }
+    if ((e = hipMemcpyAsync(musd, h_musd, sizeof(h_musd), hipMemcpyHostToDevice, c->stream))) return hip_err(c, e, "H2D norm");
+    if ((rc = features_dev(c, feat))) return rc;  // (synchronises: h_musd and core are on the device)
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((ns + 255) / 256, 8192));
+    hipLaunchKernelGGL(inputs_kernel, dim3(grid), dim3(256), 0, c->stream, (const int64_t*)feat,
+                       (const uint32_t*)c->src.ptr, (const uint32_t*)dcore, ns, c->nv, M->x);
+    hipLaunchKernelGGL(norm_kernel, dim3(std::min<uint64_t>((ns * 8 + 255) / 256, 8192)), dim3(256), 0, c->stream,
+                       M->x, ns, (const float*)musd);
+    MlpArgs a{M->params, M->x, nullptr, nullptr, ns, 1.f, nullptr, nullptr, M->out, nullptr};
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(std::min<uint64_t>((ns + 63) / 64, (uint64_t)M->num_cus * 4)), dim3(256), 0,
+                       c->stream, a);
+    hipLaunchKernelGGL(sched_keys_kernel, dim3(kb), dim3(256), 0, c->stream, (const float*)M->out, ns, keys, idx, part);
+    if ((e = hipGetLastError())) return hip_err(c, e, "schedule launch");
+    std::vector<double> h_part(kb);
+    if ((e = hipMemcpyAsync(h_part.data(), part, kb * 8, hipMemcpyDeviceToHost, c->stream))) return hip_err(c, e, "D2H costs");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "schedule sync");
+    double sum = 0;
+    for (uint64_t b = 0; b < kb; b++) sum += h_part[b];
+    const double share = std::max(sum / (double)std::max<uint64_t>(waves, 1), 1.0);
+    hipLaunchKernelGGL(sched_split_kernel, dim3(grid), dim3(256), 0, c->stream, (const float*)M->out,
+                       (const int64_t*)feat, ns, share, max_split, dsplit);
+    size_t tb = t_sort;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(base + o_temp, tb, keys, keys2, idx, dorder, (int)ns, 0, 32, c->stream)))
+        return hip_err(c, e, "order sort");
+    if ((e = hipGetLastError())) return hip_err(c, e, "split launch");
+    if ((e = hipMemcpyAsync(order, dorder, ns * 4, hipMemcpyDeviceToHost, c->stream)) ||
+        (e = hipMemcpyAsync(split, dsplit, ns * 4, hipMemcpyDeviceToHost, c->stream)) ||
+        (pred && (e = hipMemcpyAsync(pred, M->out, ns * 4, hipMemcpyDeviceToHost, c->stream))))
+        return hip_err(c, e, "D2H schedule");
+    if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "schedule sync");
     return G2_OK;
 }
 

@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
             uint32_t hb_p = 1;  // highest power of two <= p0
             while ((uint64_t)hb_p * 2 <= p0) hb_p *= 2;
             for (uint64_t k0 = 0; k0 < d; k0 += kWave) {
-                const uint32_t nk = (uint32_t)min<uint64_t>(kWave, d - k0);
+                const uint32_t nk = (uint32_t)umin64(kWave, d - k0);
                 uint32_t len = 0;
                 if (lane < nk) {  // (a row of at most 64 entries is streamed whole)
                     const uint64_t k = k0 + lane;
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
             // neighbour's row slice (located in C by binary search) or one
             // position of C (looked up in the slice), whichever is cheaper.
             for (uint64_t j0 = 0; j0 < p0; j0 += kWave) {
-                const uint32_t nj = (uint32_t)min<uint64_t>(kWave, p0 - j0);
+                const uint32_t nj = (uint32_t)umin64(kWave, p0 - j0);
                 uint32_t len = 0;
                 if (lane < nj) {
                     const uint32_t p = C[s + j0 + lane];
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                 aP = L;
                 aX = LX;
                 lfr = LX + p0 * WX2;
-                DL = min<uint64_t>(nfr, p0 + 1);
+                DL = umin64(nfr, p0 + 1);
                 sE = s2;
                 WXE = WX2;
                 FWE = FW2;
@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
         // holds the cliques of size i
         my_cliques += small_cliques;
         if (small_hist) atomicAdd(&lhist[lane], (unsigned long long)small_hist);
-        if (small_big) atomicAdd(&a.hist[min<uint32_t>(kWave + lane, a.hist_len - 1)], (unsigned long long)small_big);
+        if (small_big) atomicAdd(&a.hist[umin32(kWave + lane, a.hist_len - 1)], (unsigned long long)small_big);
     }
     if (lane == 0) {
         if (my_cliques) atomicAdd(&a.ctr[CTR_CLIQUES], (unsigned long long)my_cliques);
@@ -1201,7 +1201,7 @@ __global__ __launch_bounds__(256) void mce_group_kernel(MceArgs a) {
     my_nodes = wave_sum(my_nodes);
     my_cliques = wave_sum(my_cliques);
     if (hcount) atomicAdd(&lhist[gl], (unsigned long long)hcount);
-    if (hbig) atomicAdd(&a.hist[min<uint32_t>(G + gl, a.hist_len - 1)], (unsigned long long)hbig);
+    if (hbig) atomicAdd(&a.hist[umin32(G + gl, a.hist_len - 1)], (unsigned long long)hbig);
     if (lane == 0) {
         if (my_cliques) atomicAdd(&a.ctr[CTR_CLIQUES], (unsigned long long)my_cliques);
         if (my_nodes) atomicAdd(&a.ctr[CTR_NODES], (unsigned long long)my_nodes);

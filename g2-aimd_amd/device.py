@@ -30,7 +30,7 @@ EXPORTS = ("g2_create", "g2_set_tuning", "g2_destroy", "g2_last_error", "g2_uplo
            "g2_pinned_free", "g2_plan_local_eligible", "g2_features", "g2_mlp_create",
            "g2_mlp_destroy", "g2_mlp_get_params", "g2_mlp_set_params", "g2_mlp_infer", "g2_mlp_grad",
            "g2_mlp_train", "g2_gm_run_batch", "g2_gm_autotune", "g2_core_numbers", "g2_root_inputs",
-           "g2_gm_run_noalloc", "g2_noalloc_last_error")
+           "g2_gm_run_noalloc", "g2_noalloc_last_error", "g2_learned_schedule")
 G2_NFEAT_INT = 5
 G2_MLP_NPARAM = 4801
 
@@ -114,6 +114,7 @@ def hip_lib() -> C.CDLL:
     L.g2_mlp_grad.argtypes = [vp, f32p, f32p, C.c_uint64, f32p, C.POINTER(C.c_float)]
     L.g2_mlp_train.argtypes = [vp, f32p, f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
                                C.POINTER(C.c_float)]
+    L.g2_learned_schedule.argtypes = [vp, vp, f32p, f32p, vp, C.c_uint64, C.c_uint32, u32p, u32p, vp]
     _hip = L
     return L
 
@@ -294,6 +295,20 @@ class Executor:
         self._check(hip_lib().g2_root_inputs(self._ctx, cr.ctypes.data if cr is not None else None, out),
                     "g2_root_inputs")
         return out[: self.ns]
+
+    def learned_schedule(self, mlp, mu, sd, waves: int, core=None, max_split: int = 0):
+        """(order, split, pred) of the uploaded sources from the cost model,
+        computed on the device (g2_learned_schedule)."""
+        n = max(self.ns, 1)
+        order = np.zeros(n, np.uint32)
+        split = np.zeros(n, np.uint32)
+        pred = np.zeros(n, np.float32)
+        cr = None if core is None else np.ascontiguousarray(core, dtype=np.uint32)
+        self._check(hip_lib().g2_learned_schedule(
+            self._ctx, mlp._h, np.ascontiguousarray(mu, dtype=np.float32), np.ascontiguousarray(sd, dtype=np.float32),
+            cr.ctypes.data if cr is not None else None, waves, max_split, order, split, pred.ctypes.data),
+            "g2_learned_schedule")
+        return order[: self.ns], split[: self.ns], pred[: self.ns]
 
     def features(self) -> np.ndarray:
         """Integer root features (ns x 5): degree, 2-paths, triangles,

@@ -250,6 +250,19 @@ int  g2_mlp_grad(g2_mlp* mlp, const float* x, const float* y, uint64_t n, float*
 int  g2_mlp_train(g2_mlp* mlp, const float* x, const float* y, uint64_t n, uint32_t epochs,
                   uint32_t batch, float lr, float* loss);
 
+/* The learned schedule of the uploaded view, computed on the device: the
+ * features, the 8 inputs standardised as (x - mu[k]) / sd[k] in fp64, the
+ * cost model's predictions pred (log(1 + subtree nodes)), then order = the
+ * sources longest predicted first (cost = expm1(clamp(pred, 0, 60)), ties
+ * in source order) and split = per source ceil(cost / share) parts, at most
+ * its later neighbours and max_split (0: 4096), at least 1, with share =
+ * max(sum of costs / waves, 1) -- g2h_learned_schedule's rule.  core: host,
+ * nv entries (NULL: g2_core_numbers of the view).  order, split (and pred if
+ * not NULL): host, ns entries; they feed g2_sched.  mlp must be on the
+ * context's device. */
+int  g2_learned_schedule(g2_ctx* ctx, g2_mlp* mlp, const float* mu, const float* sd, const uint32_t* core,
+                         uint64_t waves, uint32_t max_split, uint32_t* order, uint32_t* split, float* pred);
+
 /* Device properties used for launch sizing */
 int  g2_device_info(int device, uint32_t* num_cus, uint64_t* total_mem, char* name, uint32_t name_len);
 /* Number of visible GPUs (cudaGetDeviceCount in system/worker.h:45-48). */

@@ -269,3 +269,35 @@ def test_device_core_numbers_and_inputs(g2, dev):
                         f[:, 3].astype(f32) / np.maximum(d, f32(1)), np.log1p(f[:, 4].astype(f32))], axis=1)
         assert ref.dtype == np.float32
         assert int(_ulps(x, ref).max()) <= 1
+
+
+def test_device_learned_schedule(g2, dev):
+    """g2_learned_schedule keeps the whole schedule on the device: its
+    predictions equal the cost model run on the host-standardised device
+    inputs bit for bit, its order is the stable longest-predicted-first sort
+    of them, and its parts are g2h_learned_schedule's (host C++) rule; the
+    same holds for the sources of a hop-2 view."""
+    print("case: learned schedule on the device", flush=True)
+    S = importlib.import_module("g2-aimd_amd.sched")
+    g, core = social(g2, S, 200_000, 0x6732A1D0C0000311)
+    m = dev.Mlp(0, seed=5)
+    waves = 256 * 16
+    with dev.Executor(device=0) as ex:
+        cases = [(g, None), (None, np.nonzero(np.arange(g.nv) % 3 == 1)[0].astype(np.uint32))]
+        for view, src in cases:
+            if src is None:
+                ex.transfer(g)
+            else:
+                ex.transfer(g2.view_materialize(g, src, 2), sources=src)
+            x = ex.root_inputs(core)
+            norm = S.Standardizer(x)
+            host_pred = m.infer(norm(x))
+            order, split, pred = ex.learned_schedule(m, norm.mu, norm.sd, waves, core=core)
+            assert np.array_equal(pred, host_pred)
+            p = np.where(np.isfinite(pred), np.clip(pred, 0, 60), 0).astype(np.float32)
+            assert np.array_equal(order, np.argsort(~p.view(np.uint32), kind="stable"))
+            h_order, h_split = S.learned_schedule(pred, ex.features()[:, 4], waves)
+            assert np.array_equal(order, h_order)
+            assert np.array_equal(split, h_split)
+            print(f"ns={len(order)}: {int((split > 1).sum())} split roots, max parts {int(split.max())}", flush=True)
+    m.close()
