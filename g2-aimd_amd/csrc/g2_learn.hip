@@ -2,9 +2,11 @@
 // scheduler (no counterpart in the reference, whose roots run in source order).
 //
 // * feature_kernel: one wave64 per root computes integer features
-//     deg(v), 2-path count sum_{u in N(v)} (deg(u)-1), triangles t(v),
-//     neighbours sharing >= 1 common neighbour with v, and p0 = |{u > v}|
-//   (the size of the root's P set in the maximal-clique search tree);
+//     deg(v), the distinct 2-hop ball size |{v} u N(v) u N(N(v))| (a
+//     HyperLogLog estimate from the registers sketch_kernel leaves per
+//     vertex), triangles t(v), neighbours sharing >= 1 common neighbour with
+//     v, and p0 = |{u > v}| (the size of the root's P set in the
+//     maximal-clique search tree);
 //   triangles and triangle-carrying neighbours come from one pass over the
 //   view's oriented rows (local_triangle_features: each triangle found once,
 //   at its lowest (degree, id) rank) when every source's neighbours have
@@ -45,7 +47,46 @@ struct FeatArgs {
     const unsigned long long* tri;
     const uint32_t* sup;
     const uint32_t* rank;
+    const uint8_t* sketch;  // nv x 64 HyperLogLog registers of the closed rows N[u]
+    double lc[65];          // linear counting: lc[V] = 64 ln(64 / V), from the host's libm
 };
+
+// HyperLogLog of vertex sets (Flajolet, Fusy, Gandouet, Meunier 2007) with
+// m = 64 registers: hash = the splitmix64 finaliser of the id, register =
+// its top 6 bits, rank = leading zeros of the other 58 bits + 1, capped at 32
+// so that sum 2^(32 - reg) is an exact integer.
+__device__ inline uint64_t hll_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ inline void hll_put(uint32_t* reg, uint32_t w) {
+    const uint64_t h = hll_mix(w);
+    const uint64_t rest = h << 6;
+    const uint32_t rho = rest ? umin32((uint32_t)__builtin_clzll(rest) + 1, 32u) : 32u;
+    atomicMax(&reg[h >> 58], rho);
+}
+
+// registers of every vertex's closed row N[u] = N(u) u {u}: one wave per
+// vertex, the maxima taken in the wave's 64 LDS words
+__global__ __launch_bounds__(256) void sketch_kernel(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols,
+                                                     uint64_t nv, uint8_t* __restrict__ sk) {
+    __shared__ uint32_t regs[4][kWave];
+    const uint32_t lane = lane_id();
+    uint32_t* reg = regs[threadIdx.x / kWave];
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t u = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < nv; u += nw) {
+        reg[lane] = 0;
+        wave_sync();
+        if (lane == 0) hll_put(reg, (uint32_t)u);
+        const uint64_t lo = rp[u], hi = rp[u + 1];
+        for (uint64_t e = lo + lane; e < hi; e += kWave) hll_put(reg, cols[e]);
+        wave_sync();
+        sk[u * kWave + lane] = (uint8_t)reg[lane];
+        wave_sync();
+    }
+}
 
 __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
     const uint32_t lane = lane_id();
@@ -62,12 +103,23 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
         const uint64_t ti = t++;
         const uint32_t v = uniform(a.src[ti]);
         const uint64_t lo = a.rp[v], hi = a.rp[v + 1], d = hi - lo;
-        uint64_t two = 0, p0 = 0, tri2 = 0, shared = 0;
-        for (uint64_t e = lo + lane; e < hi; e += kWave) {
-            const uint32_t u = a.cols[e];
-            two += (a.rp[u + 1] - a.rp[u]) - 1;
-            p0 += u > v;
+        uint64_t p0 = 0, tri2 = 0, shared = 0;
+        for (uint64_t e = lo + lane; e < hi; e += kWave) p0 += a.cols[e] > v;
+        // 2-hop ball: register-wise maximum over N[v] and the N[u], u in N(v)
+        // (lane l holds register l; 8 neighbours' registers in flight)
+        uint32_t reg = a.sketch[(uint64_t)v * kWave + lane];
+        for (uint64_t e0 = lo; e0 < hi; e0 += 8) {
+            uint32_t r[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint64_t e = e0 + q;
+                r[q] = e < hi ? a.sketch[(uint64_t)uniform(a.cols[e]) * kWave + lane] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) reg = umax32(reg, r[q]);
         }
+        const uint64_t hsum = wave_sum(1ull << (32 - reg));
+        const uint64_t zeros = wave_sum(reg == 0);
         for (uint64_t e = a.tri ? hi : lo; e < hi; e++) {
             const uint32_t u = a.cols[e];
             const uint64_t ulo = a.rp[u], uhi = a.rp[u + 1];
@@ -81,12 +133,15 @@ __global__ __launch_bounds__(256) void feature_kernel(FeatArgs a) {
             tri2 += c;
             shared += c > 0;
         }
-        two = wave_sum(two);
         p0 = wave_sum(p0);
         if (lane == 0) {
+            // E = alpha_64 m^2 / sum 2^-reg; linear counting below 5m/2 when a
+            // register is empty
+            double est = (0.709 * 4096.0 * 4294967296.0) / (double)hsum;
+            if (est <= 160.0 && zeros > 0) est = a.lc[zeros];
             int64_t* o = a.out + ti * G2_NFEAT_INT;
             o[0] = (int64_t)d;
-            o[1] = (int64_t)two;
+            o[1] = (int64_t)(est + 0.5);
             o[2] = a.tri ? (int64_t)a.tri[a.rank[v]] : (int64_t)(tri2 / 2);
             o[3] = a.tri ? (int64_t)a.sup[a.rank[v]] : (int64_t)shared;
             o[4] = (int64_t)p0;
@@ -157,7 +212,7 @@ static int core_numbers_run(g2_ctx* c, uint32_t* dcore, uint8_t* act0, uint8_t* 
     const uint64_t wgrid = std::max<uint64_t>(1, (uint64_t)c->num_cus * 32 / 4);
     auto* changed = (unsigned long long*)c->counters.ptr;
     hipLaunchKernelGGL(core_init_kernel, dim3(grid), dim3(256), 0, c->stream, rp, nv, dcore, act0);
-    for (int it = 0;; it++) {
+    for (;;) {
         hipLaunchKernelGGL(core_clear_kernel, dim3(grid), dim3(256), 0, c->stream, act1, nv);
         if ((e = hipMemsetAsync(changed, 0, 8, c->stream))) return hip_err(c, e, "memset");
         hipLaunchKernelGGL(core_step_kernel, dim3(wgrid), dim3(256), 0, c->stream, rp, cols, nv, dcore, act0, act1, changed);
@@ -277,10 +332,25 @@ static int features_dev(g2_ctx* c, int64_t* dfeat) {
             }
         }
     }
+    // HyperLogLog registers of every vertex's closed row (nv x 64 B, for this call)
+    struct SketchBuf {
+        g2_ctx* c;
+        DevBuf b;
+        ~SketchBuf() {
+            if (b.ptr) (void)hipFree(b.ptr);
+            c->used -= b.bytes;
+        }
+    } sk_buf{c, {}};
+    if ((rc = ensure(c, sk_buf.b, nv1 * kWave, "2-hop sketches"))) return rc;
+    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
+    if (c->nv) hipLaunchKernelGGL(sketch_kernel, dim3(blocks), dim3(256), 0, c->stream, (const uint64_t*)c->rp.ptr,
+                                  (const uint32_t*)c->cols.ptr, c->nv, (uint8_t*)sk_buf.b.ptr);
+    if ((e = hipGetLastError())) return hip_err(c, e, "sketch launch");
     if ((e = hipMemsetAsync(c->counters.ptr, 0, 8, c->stream))) return hip_err(c, e, "memset");
     FeatArgs a{(const uint64_t*)c->rp.ptr, (const uint32_t*)c->cols.ptr, (const uint32_t*)c->src.ptr, c->ns,
-               (unsigned long long*)c->counters.ptr, dfeat, tri, sup, rank};
-    const uint64_t blocks = std::max<uint64_t>(1, (uint64_t)c->num_cus * waves_per_cu(c, WK_FEATURES) / 4);
+               (unsigned long long*)c->counters.ptr, dfeat, tri, sup, rank, (const uint8_t*)sk_buf.b.ptr, {}};
+    a.lc[0] = 0.0;
+    for (int z = 1; z <= 64; z++) a.lc[z] = 64.0 * std::log(64.0 / (double)z);
     hipLaunchKernelGGL(feature_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
     if ((e = hipGetLastError())) return hip_err(c, e, "feature launch");
     if ((e = hipStreamSynchronize(c->stream))) return hip_err(c, e, "feature sync");

@@ -311,8 +311,9 @@ class Executor:
         return order[: self.ns], split[: self.ns], pred[: self.ns]
 
     def features(self) -> np.ndarray:
-        """Integer root features (ns x 5): degree, 2-paths, triangles,
-        neighbours sharing a neighbour with the root, later neighbours."""
+        """Integer root features (ns x 5): degree, distinct 2-hop ball size
+        (HyperLogLog estimate), triangles, neighbours sharing a neighbour with
+        the root, later neighbours."""
         out = np.zeros((max(self.ns, 1), G2_NFEAT_INT), np.int64)
         self._check(hip_lib().g2_features(self._ctx, out), "g2_features")
         return out[: self.ns]

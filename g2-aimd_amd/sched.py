@@ -56,7 +56,7 @@ def _check(rc, what):
     if rc != 0:
         raise RuntimeError(f"{what} failed ({rc}): {host_lib().g2h_last_error().decode()}")
 
-FEATURE_NAMES = ("log1p_degree", "log1p_core", "log1p_2paths", "log1p_triangles", "clustering",
+FEATURE_NAMES = ("log1p_degree", "log1p_core", "log1p_2hop_ball", "log1p_triangles", "clustering",
                  "degeneracy_rank", "shared_neighbour_fraction", "log1p_later_neighbours")
 
 

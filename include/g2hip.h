@@ -215,8 +215,11 @@ int  g2_plan_local_eligible(const g2_plan* plan);
 /* ---- learned root scheduler (no counterpart in the reference) ----------- */
 
 /* Integer features of every uploaded source, G2_NFEAT_INT per source:
- * degree, 2-path count sum_{u in N(v)} (deg(u)-1), triangles at v, neighbours
- * sharing a common neighbour with v, and |{u in N(v): u > v}| (the root's P). */
+ * degree, the distinct 2-hop ball size |{v} u N(v) u N(N(v))| (HyperLogLog
+ * estimate, 64 registers: rounded to the nearest integer, relative standard
+ * error about 13%; over a view, N(u) is the row the view holds), triangles at
+ * v, neighbours sharing a common neighbour with v, and |{u in N(v): u > v}|
+ * (the root's P).  Temporary device memory: 64 B per vertex of the view. */
 #define G2_NFEAT_INT 5
 int  g2_features(g2_ctx* ctx, int64_t* out);
 
@@ -226,7 +229,7 @@ int  g2_features(g2_ctx* ctx, int64_t* out);
 int  g2_core_numbers(g2_ctx* ctx, uint32_t* core);
 /* The cost model's 8 inputs of every uploaded source (ns x 8 fp32, source
  * order), computed on the device with one fp32 rounding per operation:
- * log1p(degree), log1p(core number), log1p(2-paths), log1p(triangles),
+ * log1p(degree), log1p(core number), log1p(2-hop ball), log1p(triangles),
  * local clustering 2t / (d(d-1)), id / nv (degeneracy rank of a recoded
  * graph), shared-neighbour fraction, log1p(later neighbours).  core (host,
  * nv entries) gives the core numbers; NULL computes them from the view

@@ -158,7 +158,7 @@ const char* g2h_aimd_adjust(g2h_aimd* a, int overflow);
 /* core number of every vertex (bucket peel, any CSR with sorted rows) */
 int  g2h_core_numbers(const uint64_t* row_ptrs, uint64_t nv, const uint32_t* cols, uint32_t* core);
 /* the cost model's 8 inputs of ns roots, float32 row-major: log1p(degree),
- * log1p(core number), log1p(2-paths), log1p(triangles), local clustering
+ * log1p(core number), log1p(2-hop ball size), log1p(triangles), local clustering
  * 2t/(d(d-1)), id/nv (degeneracy rank of a kcore-recoded graph), shared-
  * neighbour fraction, log1p(later neighbours); feat = g2_features' ns x 5 */
 int  g2h_root_features(const int64_t* feat, const uint32_t* sources, uint64_t ns, const uint32_t* core,

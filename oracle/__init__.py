@@ -92,6 +92,9 @@ def bind(path: str) -> C.CDLL:
     L.orc_kclique_count_ranked.argtypes = [u64p, C.c_uint64, u32p, C.c_uint32, u32p, C.c_uint64,
                                            C.c_int, P64]
     L.orc_local_bytes.argtypes = [u64p, C.c_uint64, u32p, u32p, C.c_uint64, C.c_uint32, u64p]
+    i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    for f in (L.orc_two_hop, L.orc_two_hop_exact):
+        f.argtypes = [u64p, C.c_uint64, u32p, u32p, C.c_uint64, i64p]
     return L
 
 
@@ -186,3 +189,22 @@ def local_bytes(g, n: int, sources=None) -> np.ndarray:
     rc = lib().orc_local_bytes(g.row_ptrs, g.nv, _cols(g), s, len(src), n, out)
     assert rc == 0, rc
     return out
+
+
+def two_hop(g, sources=None) -> np.ndarray:
+    """HyperLogLog estimate (64 registers) of the 2-hop ball size |{v} u N(v) u N(N(v))|
+    per source, the integers the device feature kernel must reproduce."""
+    src = _src(sources, g.nv)
+    out = np.zeros(max(len(src), 1), np.int64)
+    s = src if len(src) else np.zeros(1, np.uint32)
+    assert lib().orc_two_hop(g.row_ptrs, g.nv, _cols(g), s, len(src), out) == 0
+    return out[: len(src)]
+
+
+def two_hop_exact(g, sources=None) -> np.ndarray:
+    """exact 2-hop ball size per source (brute force)"""
+    src = _src(sources, g.nv)
+    out = np.zeros(max(len(src), 1), np.int64)
+    s = src if len(src) else np.zeros(1, np.uint32)
+    assert lib().orc_two_hop_exact(g.row_ptrs, g.nv, _cols(g), s, len(src), out) == 0
+    return out[: len(src)]

@@ -53,6 +53,7 @@
 // ============================================================================
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -387,6 +388,68 @@ int orc_gm(const uint64_t* rp, uint64_t nv, const uint32_t* cols, uint64_t ne,
 }
 
 // ---------------------------------------------------------------------------
+// Distinct 2-hop ball size per source, the cost model's "2-hop frontier"
+// input: a HyperLogLog estimate (Flajolet, Fusy, Gandouet, Meunier 2007) of
+// |B2(v)| = |{v} u N(v) u N(N(v))|, with m = 64 registers, the splitmix64
+// finaliser as hash, register index = top 6 bits, rank = leading zeros of the
+// other 58 bits + 1 capped at 32.  Each vertex u gets the registers of its
+// closed row N[u] = N(u) u {u}; a source's registers are the register-wise
+// maximum over N[v] and every N[u], u in N(v).  Estimate: E = alpha m^2 /
+// sum 2^-reg (alpha_64 = 0.709), evaluated as alpha m^2 2^32 / sum 2^(32-reg)
+// with an exact integer sum; when E <= 5m/2 and V registers are zero (V > 0),
+// linear counting m ln(m/V) instead.  Output: (int64)(E + 0.5).  No reference
+// counterpart (the reference has no root features); the device kernel
+// computes the same integers and doubles.
+static inline uint64_t hll_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static inline void hll_add(uint8_t* reg, uint32_t w) {
+    const uint64_t h = hll_mix(w);
+    const uint32_t idx = (uint32_t)(h >> 58);
+    const uint64_t rest = h << 6;
+    const uint32_t rho = rest ? std::min<uint32_t>((uint32_t)__builtin_clzll(rest) + 1, 32u) : 32u;
+    reg[idx] = std::max<uint8_t>(reg[idx], (uint8_t)rho);
+}
+static inline int64_t hll_estimate(const uint8_t* reg) {
+    uint64_t sum = 0;
+    uint32_t zeros = 0;
+    for (int l = 0; l < 64; l++) {
+        sum += 1ull << (32 - reg[l]);
+        zeros += reg[l] == 0;
+    }
+    double e = (0.709 * 4096.0 * 4294967296.0) / (double)sum;
+    if (e <= 160.0 && zeros > 0) e = 64.0 * std::log(64.0 / (double)zeros);
+    return (int64_t)(e + 0.5);
+}
+
+int orc_two_hop(const uint64_t* rp, uint64_t nv, const uint32_t* cols, const uint32_t* sources, uint64_t ns,
+                int64_t* out) {
+    for (uint64_t i = 0; i < ns; i++)
+        if (sources[i] >= nv) return -2;
+    std::vector<uint8_t> sk(nv * 64, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t u = 0; u < (int64_t)nv; u++) {
+        uint8_t* r = &sk[(uint64_t)u * 64];
+        hll_add(r, (uint32_t)u);
+        for (uint64_t e = rp[u]; e < rp[u + 1]; e++) hll_add(r, cols[e]);
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)ns; i++) {
+        const uint32_t v = sources[i];
+        uint8_t r[64];
+        std::memcpy(r, &sk[(uint64_t)v * 64], 64);
+        for (uint64_t e = rp[v]; e < rp[v + 1]; e++) {
+            const uint8_t* q = &sk[(uint64_t)cols[e] * 64];
+            for (int l = 0; l < 64; l++) r[l] = std::max(r[l], q[l]);
+        }
+        out[i] = hll_estimate(r);
+    }
+    return 0;
+}
+
 // Independent checkers (brute force), used to pin the restatements above.
 // ---------------------------------------------------------------------------
 
@@ -639,6 +702,33 @@ int orc_local_bytes(const uint64_t* rp, uint64_t nv, const uint32_t* cols, const
         acc[pass] += b;
     }
     for (int k = 0; k < 5; k++) per_pass[k] = acc[k];
+    return 0;
+}
+
+// Exact |B2(v)| per source (brute force with a marker array), the value the
+// estimate above is checked against.
+int orc_two_hop_exact(const uint64_t* rp, uint64_t nv, const uint32_t* cols, const uint32_t* sources, uint64_t ns,
+                      int64_t* out) {
+    for (uint64_t i = 0; i < ns; i++)
+        if (sources[i] >= nv) return -2;
+    std::vector<uint64_t> mark(nv, ~0ull);
+    for (uint64_t i = 0; i < ns; i++) {
+        const uint32_t v = sources[i];
+        int64_t c = 0;
+        auto touch = [&](uint32_t w) {
+            if (mark[w] != i) {
+                mark[w] = i;
+                c++;
+            }
+        };
+        touch(v);
+        for (uint64_t e = rp[v]; e < rp[v + 1]; e++) {
+            const uint32_t u = cols[e];
+            touch(u);
+            for (uint64_t f = rp[u]; f < rp[u + 1]; f++) touch(cols[f]);
+        }
+        out[i] = c;
+    }
     return 0;
 }
 

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import graphs as G
+import oracle
 from oracle import mlp_ref
 
 pytestmark = pytest.mark.gpu
@@ -27,7 +28,7 @@ def test_features(g2, gpu_exec):
         deg = A.sum(1)
         A2 = A @ A
         tri = (A2 * A).sum(1) // 2
-        two = A @ deg - deg
+        two = oracle.two_hop(g)
         shared = ((A2 * A) > 0).sum(1)
         p0 = np.array([np.sum(g.cols[g.row_ptrs[v]:g.row_ptrs[v + 1]] > v) for v in range(n)])
         assert np.array_equal(f[:, 0], deg)
@@ -38,8 +39,9 @@ def test_features(g2, gpu_exec):
 
 
 def features_ref(g):
-    """deg, 2-paths, triangles, triangle-carrying neighbours, later neighbours
-    of every vertex, from scipy's sparse products."""
+    """deg, 2-hop ball estimate (the oracle's HyperLogLog restatement),
+    triangles, triangle-carrying neighbours, later neighbours of every vertex,
+    the counts from scipy's sparse products."""
     import scipy.sparse as sp
     n = g.nv
     src = np.repeat(np.arange(n), np.diff(g.row_ptrs).astype(np.int64))
@@ -48,7 +50,7 @@ def features_ref(g):
     T = (A @ A).multiply(A)          # T[v, u] = common neighbours of the edge (v, u)
     tri = np.asarray(T.sum(1)).ravel() // 2
     shared = np.asarray((T > 0).sum(1)).ravel()
-    two = A @ deg - deg
+    two = oracle.two_hop(g)
     p0 = np.asarray((sp.triu(A, 1) > 0).sum(1)).ravel()
     return np.stack([deg, two, tri, shared, p0], axis=1)
 
@@ -72,6 +74,10 @@ def test_features_oriented_and_views(g2, gpu_exec):
     gpu_exec.transfer(g2.view_materialize(g, src, 1), sources=src)
     f = gpu_exec.features()
     assert np.array_equal(f[:, [0, 4]], ref[src][:, [0, 4]])
+    # a hop-1 view has no rows for its sources' neighbours: their closed rows
+    # are themselves, so the ball is estimated over the closed neighbourhood
+    v1 = g2.view_materialize(g, src, 1)
+    assert np.array_equal(f[:, 1], oracle.two_hop(v1, src))
 
 
 def test_mlp_forward_and_grad(dev):

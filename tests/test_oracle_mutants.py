@@ -4,8 +4,8 @@ Each case below plants one plausible mistake in a copy of oracle/oracle.cpp
 (a dropped term, a wrong label, an inverted test, an off-by-one), builds that
 copy on the CPU, and runs the pinning tests of test_oracle_mce.py or
 test_oracle_gm.py against it.  At least one of them must fail.  The mistakes
-are planted only in the restatements (orc_mce, orc_gm and their helpers), never
-in the independent checkers they are compared with.  The cost model's CPU
+are planted only in the restatements (orc_mce, orc_gm, orc_two_hop and their
+helpers), never in the independent checkers they are compared with.  The cost model's CPU
 reference (oracle/mlp_ref.py) gets the same treatment against test_mlp_ref.py.
 """
 import importlib.util
@@ -16,6 +16,7 @@ import subprocess
 import pytest
 
 import test_mlp_ref as TL
+import test_oracle_features as TF
 import test_oracle_gm as TG
 import test_oracle_mce as TM
 
@@ -118,12 +119,33 @@ MUTANTS = [
     ("gm_bytes_expand_priced_as_prefix", "gm",
      "return 4 * (uint64_t)(id + 1) * nout;",
      "return 4 * (id + nout);"),
+    # 2-hop ball estimate: rank = leading zeros + 1
+    ("twohop_rank_off_by_one", "feat",
+     "std::min<uint32_t>((uint32_t)__builtin_clzll(rest) + 1, 32u)",
+     "std::min<uint32_t>((uint32_t)__builtin_clzll(rest) + 2, 32u)"),
+    # registers of the closed row N[u] (u itself included)
+    ("twohop_open_rows", "feat",
+     "        hll_add(r, (uint32_t)u);\n",
+     "\n"),
+    # the neighbours' registers are merged into the source's
+    ("twohop_neighbours_not_merged", "feat",
+     "for (int l = 0; l < 64; l++) r[l] = std::max(r[l], q[l]);",
+     "(void)q;"),
+    # linear counting for small estimates
+    ("twohop_no_linear_counting", "feat",
+     "if (e <= 160.0 && zeros > 0)",
+     "if (e <= 0.0 && zeros > 0)"),
+    # the register index is the hash's top 6 bits, not its lowest
+    ("twohop_index_shares_rank_bits", "feat",
+     "const uint32_t idx = (uint32_t)(h >> 58);",
+     "const uint32_t idx = (uint32_t)((h << 6) >> 58);"),
 ]
 
 MCE_PINS = ["test_bruteforce", "test_complete_graph", "test_complete_bipartite", "test_cycle",
             "test_moon_moser", "test_turan", "test_disjoint_union_and_isolated",
             "test_relabel_invariance_and_root_partition", "test_per_root_nodes_tiny",
             "test_per_root_nodes_tie_rule", "test_byte_model_hand_worked"]
+FEAT_PINS = ["test_tiny_balls_near_exact", "test_star_and_complete", "test_bias_and_spread_on_skewed_graphs"]
 GM_PINS = ["test_cliques_closed_form", "test_presets_vs_bruteforce",
            "test_random_patterns_vs_bruteforce", "test_kclique_independent_method",
            "test_per_root_counts_sum", "test_byte_model_hand_worked"]
@@ -172,7 +194,7 @@ def mutant_libs(tmp_path_factory):
 @pytest.mark.parametrize("name,suite", [(m[0], m[1]) for m in MUTANTS])
 def test_pins_catch_planted_mistake(g2, orc, mutant_libs, monkeypatch, name, suite):
     monkeypatch.setattr(orc, "_lib", orc.bind(mutant_libs[name]))
-    mod, names = (TM, MCE_PINS) if suite == "mce" else (TG, GM_PINS)
+    mod, names = {"mce": (TM, MCE_PINS), "gm": (TG, GM_PINS), "feat": (TF, FEAT_PINS)}[suite]
     caught = []
     for label, fn, kw in pin_cases(mod, names):
         try:
