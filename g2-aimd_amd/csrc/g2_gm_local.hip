@@ -102,6 +102,12 @@ __host__ __device__ inline uint32_t row_stride(uint64_t m) { return (uint32_t)((
 // u64 words of a root's local rows
 __host__ __device__ inline uint64_t local_need(uint64_t m) { return m * row_stride(m); }
 
+// set bit b of a local row word (the 32-bit half holding it: a 32-bit
+// atomic takes one LDS bank where the 64-bit one takes two)
+__device__ inline void set_row_bit(uint64_t* rows, uint64_t word, uint32_t b) {
+    atomicOr((unsigned int*)&rows[word] + ((b >> 5) & 1u), 1u << (b & 31u));
+}
+
 // bits of word w at positions in [lo, hi)
 __device__ inline uint64_t range_word(uint32_t lo, uint32_t hi, uint32_t w) {
     const uint32_t base = w * 64;
@@ -315,9 +321,9 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
                     a.flag[xlo + ib] = 1;
                     a.flag[(cj[q] & kRev) ? (uint64_t)sj[q] + rpos[q] - 1 : (uint64_t)sj[q] + item[q]] = 1;
                 } else {
-                    atomicOr((unsigned long long*)&rows[(uint64_t)ia * W + (ib >> 6)], 1ull << (ib & 63));
+                    set_row_bit(rows, (uint64_t)ia * W + (ib >> 6), ib);
                     if constexpr (MODE == 0)
-                        atomicOr((unsigned long long*)&rows[(uint64_t)ib * W + (ia >> 6)], 1ull << (ia & 63));
+                        set_row_bit(rows, (uint64_t)ib * W + (ia >> 6), ia);
                 }
             }
         }
@@ -762,9 +768,9 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
                     hit = ib < m && U[ib] == w[q];
                 }
                 if (hit) {  // (row offsets within the LDS rows fit 32 bits)
-                    atomicOr((unsigned long long*)&rows[ia * W + (ib >> 6)], 1ull << (ib & 63));
+                    set_row_bit(rows, ia * W + (ib >> 6), ib);
                     if constexpr (!CLQ)
-                        atomicOr((unsigned long long*)&rows[ib * W + (ia >> 6)], 1ull << (ia & 63));
+                        set_row_bit(rows, ib * W + (ia >> 6), ia);
                 }
             }
         }
