@@ -228,30 +228,53 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             f[q] = f0 + q * kWave + lane;
             j[q] = 0;
         }
-        // owners: the last member whose items start at or before f (every lane
-        // runs the same steps, so the lane reads see all lanes active)
+        // owners: the last member whose items start at or before f.  The
+        // round's first and last owners come from two ballots (pref is
+        // non-decreasing over the lanes); each item then searches only between
+        // them (every lane runs the same steps, so the lane reads see all
+        // lanes active), and a round inside one member needs no search
+        const uint32_t flast = umin32(f0 + kItemsW * kWave, T) - 1;
+        const uint32_t o0 = __popcll(ballot(lane < na && pref <= f0)) - 1;
+        const uint32_t o1 = __popcll(ballot(lane < na && pref <= flast)) - 1;
 #pragma unroll
-        for (uint32_t step = 32; step >= 1; step >>= 1) {
+        for (int q = 0; q < kItemsW; q++) j[q] = o0;
+        if (o1 > o0) {
+            uint32_t hb_o = 1;
+            while (hb_o * 2 <= o1 - o0) hb_o *= 2;
+            for (uint32_t step = hb_o; step >= 1; step >>= 1) {
 #pragma unroll
-            for (int q = 0; q < kItemsW; q++) {
-                const uint32_t c = j[q] + step;
-                const uint32_t pc = __shfl(pref, (int)c, kWave);
-                if (c < na && pc <= f[q]) j[q] = c;
+                for (int q = 0; q < kItemsW; q++) {
+                    const uint32_t c = j[q] + step;
+                    const uint32_t pc = __shfl(pref, (int)umin32(c, o1), kWave);
+                    if (c <= o1 && pc <= f[q]) j[q] = c;
+                }
             }
         }
         uint32_t hb_r = 0;  // largest search range among this lane's lookups
 #pragma unroll
         for (int q = 0; q < kItemsW; q++) {
-            item[q] = f[q] - __shfl(pref, (int)j[q], kWave);
-            cj[q] = __shfl(code, (int)j[q], kWave);
-            sj[q] = __shfl(st, (int)j[q], kWave);
-            const bool valid = f[q] < T;
-            if (valid && (cj[q] & kRev)) {
-                w[q] = U[a0 + j[q] + 1 + item[q]];  // the later member to look up
-                hb_r = max(hb_r, cj[q] & ~kRev);
+            if (o1 == o0) {  // (wave-uniform: one member owns the whole round)
+                item[q] = f[q] - read_lane(pref, o0);
+                cj[q] = read_lane(code, o0);
+                sj[q] = read_lane(st, o0);
             } else {
-                w[q] = valid ? a.ocols[(uint64_t)sj[q] + item[q]] : 0;
+                item[q] = f[q] - __shfl(pref, (int)j[q], kWave);
+                cj[q] = __shfl(code, (int)j[q], kWave);
+                sj[q] = __shfl(st, (int)j[q], kWave);
             }
+            const bool valid = f[q] < T, rev = valid && (cj[q] & kRev);
+            hb_r = max(hb_r, rev ? cj[q] & ~kRev : 0u);
+            // both candidates are loaded without branching (no lane waits on a
+            // load another path issued): the later member to look up, and the
+            // row entry (a reversed member's row is longer than its later-member
+            // count, so sj + item stays inside it; an invalid lane reads entry
+            // xlo, inside the array)
+            pos[q] = U[umin32(a0 + j[q] + 1 + item[q], m - 1)];
+            w[q] = a.ocols[valid ? (uint64_t)sj[q] + item[q] : xlo];
+        }
+#pragma unroll
+        for (int q = 0; q < kItemsW; q++) {
+            w[q] = f[q] >= T ? 0u : (cj[q] & kRev) ? pos[q] : w[q];
             pos[q] = 0;
         }
         uint32_t lim[kItemsW];
@@ -707,15 +730,23 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
             // w: the entry to look up in U, or for a reversed member the later
             // member to look up in the member's row (rlen > 0: that row's length)
             uint32_t rst[kItems], rlen[kItems], rmax = 0;
+            // every lane loads both candidates of all its items before any is
+            // used (no branch around a load, so no lane waits on a load that
+            // another path issued): a reversed member's row is longer than its
+            // later-member count, so rst + item stays inside it, and an invalid
+            // lane reads the root's own row
 #pragma unroll
             for (int q = 0; q < kItems; q++) {
                 const uint32_t o = own[q], code = mcode[o], item = f[q] - mpref[o];
-                const bool valid = f[q] < T, rev = valid && (code & kRev);
+                const bool valid = f[q] < T;
                 rst[q] = mstart[o];
-                rlen[q] = rev ? (code & ~kRev) : 0;
+                rlen[q] = valid && (code & kRev) ? (code & ~kRev) : 0;
                 rmax = max(rmax, rlen[q]);
-                w[q] = !valid ? ulo : rev ? U[o + 1 + item] : a.ocols[(uint64_t)rst[q] + item];
+                pos[q] = U[umin32(o + 1 + item, m - 1)];
+                w[q] = a.ocols[valid ? (uint64_t)rst[q] + item : lo];
             }
+#pragma unroll
+            for (int q = 0; q < kItems; q++) w[q] = f[q] >= T ? ulo : rlen[q] ? pos[q] : w[q];
             uint32_t lim[kItems];
 #pragma unroll
             for (int q = 0; q < kItems; q++) {  // (an entry above U_{m-1} lands in the last bucket and misses)
