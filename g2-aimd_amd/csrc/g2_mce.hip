@@ -226,6 +226,7 @@ constexpr int kMaxBlockWaves = 4;  // __launch_bounds__(256)
 constexpr int kScoreWords = 2;     // candidate words scored per round (arena path)
 constexpr int kLdsArena = 896;     // u64 words of LDS per wave for the walk of a medium root
 constexpr int kMinLdsFrames = 16; // frames that must fit next to the rows for the LDS layout
+constexpr int kBuildItems = 4;    // neighbourhood-build items per lane in flight (C in LDS)
 
 // LDS of one wave for the register-resident path
 struct SmallLds {
@@ -556,29 +557,63 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
                     const uint32_t y = __shfl_up(incl, o, kWave);
                     if (lane >= (uint32_t)o) incl += y;
                 }
-                SL.pref[lane] = incl - len;
+                const uint32_t pref = incl - len;
+                SL.pref[lane] = pref;
                 const uint32_t T = read_lane(incl, kWave - 1);
                 wave_sync();
-                for (uint32_t f = lane; f < T; f += kWave) {
-                    uint32_t lo2 = 0, hi2 = nk;
-                    while (hi2 - lo2 > 1) {
-                        const uint32_t mid = (lo2 + hi2) >> 1;
-                        if (SL.pref[mid] <= f) lo2 = mid; else hi2 = mid;
+                // kBuildItems items per lane per round, their loads issued
+                // together.  The round's first and last owners come from two
+                // ballots (pref is non-decreasing over the lanes); each item
+                // searches only between them.
+                for (uint32_t f0 = 0; f0 < T; f0 += kBuildItems * kWave) {
+                    const uint32_t flast = umin32(f0 + kBuildItems * kWave, T) - 1;
+                    const uint32_t o0 = __popcll(ballot(lane < nk && pref <= f0)) - 1;
+                    const uint32_t o1 = __popcll(ballot(lane < nk && pref <= flast)) - 1;
+                    uint32_t fi[kBuildItems], own[kBuildItems], w[kBuildItems], pos[kBuildItems];
+#pragma unroll
+                    for (int q = 0; q < kBuildItems; q++) {
+                        fi[q] = f0 + q * kWave + lane;
+                        own[q] = o0;
                     }
-                    const uint64_t k = k0 + lo2;
-                    const uint32_t w = a.cols[SL.start[lo2] + (f - SL.pref[lo2])];
-                    uint64_t pos = s;  // first P0 position with C >= w, fixed halving steps
-                    for (uint32_t step = hb_p; step >= 1; step >>= 1)
-                        if (pos + step <= d && Cb[pos + step - 1] < w) pos += step;
-                    if (pos < d && Cb[pos] == w) {
-                        const uint64_t j2 = pos - s;
-                        if (k < s) {  // X0 member k, P0 member j2
-                            atomicOr((unsigned long long*)&adjX[j2 * WX + (k >> 6)], 1ull << (k & 63));
-                            atomicOr((unsigned long long*)&adjP[k * W + (j2 >> 6)], 1ull << (j2 & 63));
-                        } else {      // P0 members j1 < j2
-                            const uint64_t j1 = k - s;
-                            atomicOr((unsigned long long*)&adjP[(s + j1) * W + (j2 >> 6)], 1ull << (j2 & 63));
-                            atomicOr((unsigned long long*)&adjP[(s + j2) * W + (j1 >> 6)], 1ull << (j1 & 63));
+                    if (o1 > o0) {
+                        uint32_t hb_o = 1;
+                        while (hb_o * 2 <= o1 - o0) hb_o *= 2;
+                        for (uint32_t step = hb_o; step >= 1; step >>= 1) {
+#pragma unroll
+                            for (int q = 0; q < kBuildItems; q++) {
+                                const uint32_t c = own[q] + step;
+                                const uint32_t pc = SL.pref[umin32(c, o1)];
+                                own[q] = (c <= o1 && pc <= fi[q]) ? c : own[q];
+                            }
+                        }
+                    }
+                    // (an item past T reads the root's first entry and never hits)
+#pragma unroll
+                    for (int q = 0; q < kBuildItems; q++)
+                        w[q] = a.cols[fi[q] < T ? SL.start[own[q]] + (fi[q] - SL.pref[own[q]]) : lo];
+#pragma unroll
+                    for (int q = 0; q < kBuildItems; q++) pos[q] = (uint32_t)s;
+                    for (uint32_t step = hb_p; step >= 1; step >>= 1) {  // first P0 position with C >= w
+#pragma unroll
+                        for (int q = 0; q < kBuildItems; q++) {
+                            const uint32_t c = pos[q] + step;
+                            const uint32_t cv = Cb[umin32(c, (uint32_t)d) - 1];
+                            pos[q] = (c <= d && cv < w[q]) ? c : pos[q];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < kBuildItems; q++) {
+                        const uint32_t cv = Cb[umin32(pos[q], (uint32_t)d - 1)];
+                        if (fi[q] < T && pos[q] < d && cv == w[q]) {
+                            const uint64_t k = k0 + own[q], j2 = pos[q] - s;
+                            if (k < s) {  // X0 member k, P0 member j2
+                                atomicOr((unsigned long long*)&adjX[j2 * WX + (k >> 6)], 1ull << (k & 63));
+                                atomicOr((unsigned long long*)&adjP[k * W + (j2 >> 6)], 1ull << (j2 & 63));
+                            } else {      // P0 members j1 < j2
+                                const uint64_t j1 = k - s;
+                                atomicOr((unsigned long long*)&adjP[(s + j1) * W + (j2 >> 6)], 1ull << (j2 & 63));
+                                atomicOr((unsigned long long*)&adjP[(s + j2) * W + (j1 >> 6)], 1ull << (j1 & 63));
+                            }
                         }
                     }
                 }
