@@ -571,6 +571,9 @@ def main():
     traffic = args.traffic_bytes
     if traffic is None and args.scale == 22:
         traffic = profiled_traffic("gm_local_kernel_q24_rmat22.md", f"traffic_bytes_per_launch_pass{dom + 1}")
+    if achieved is not None and not 0 < achieved / HBM_PEAK_GBS <= 1:
+        raise RuntimeError(f"roofline fraction {achieved / HBM_PEAK_GBS:.3f} is outside (0, 1]: the byte model "
+                           "does not describe the timed pass")
     value = cnt / (ms_step / 1e3)
     out = {
         "metric": METRIC,
