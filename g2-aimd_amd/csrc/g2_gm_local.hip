@@ -266,11 +266,10 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
             hb_r = max(hb_r, rev ? cj[q] & ~kRev : 0u);
             // both candidates are loaded without branching (no lane waits on a
             // load another path issued): the later member to look up, and the
-            // row entry (a reversed member's row is longer than its later-member
-            // count, so sj + item stays inside it; an invalid lane reads entry
-            // xlo, inside the array)
+            // row entry (a reversed or invalid lane reads entry xlo, inside the
+            // array, instead of a row entry it does not use)
             pos[q] = U[umin32(a0 + j[q] + 1 + item[q], m - 1)];
-            w[q] = a.ocols[valid ? (uint64_t)sj[q] + item[q] : xlo];
+            w[q] = a.ocols[valid && !rev ? (uint64_t)sj[q] + item[q] : xlo];
         }
 #pragma unroll
         for (int q = 0; q < kItemsW; q++) {
