@@ -227,6 +227,7 @@ constexpr int kScoreWords = 2;     // candidate words scored per round (arena pa
 constexpr int kLdsArena = 896;     // u64 words of LDS per wave for the walk of a medium root
 constexpr int kMinLdsFrames = 16; // frames that must fit next to the rows for the LDS layout
 constexpr int kBuildItems = 4;    // neighbourhood-build items per lane in flight (C in LDS)
+constexpr uint32_t kPB = 255;      // value buckets over a root's P0 part in the build (C in LDS)
 
 // LDS of one wave for the register-resident path
 struct SmallLds {
@@ -537,6 +538,31 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
             // found from both ends sets the same bits twice.
             uint32_t hb_p = 1;  // highest power of two <= p0
             while ((uint64_t)hb_p * 2 <= p0) hb_p *= 2;
+            // Value buckets over C's P0 part when it has more than 32
+            // entries: bucket(x) = floor((x - P0[0]) * kPB / range) by one high
+            // multiply (the identity when P0 spans at most kPB ids), monotone,
+            // so a lookup searches one bucket.  pbkt (kPB + 1 u16 in the
+            // register path's row area, unused here) holds each bucket's first
+            // P0 position.
+            const bool pbk = p0 > 32;
+            uint16_t* const pbkt = reinterpret_cast<uint16_t*>(SL.adjP);
+            uint32_t plo = 0, pmul = 0, pmask = 0, hb_b = 1;
+            if (pbk) {
+                plo = Cb[s];
+                const uint32_t range = Cb[d - 1] - plo + 1;
+                pmask = __builtin_amdgcn_readfirstlane(range <= kPB ? ~0u : 0u);
+                pmul = __builtin_amdgcn_readfirstlane(range <= kPB ? 0u : (uint32_t)(((uint64_t)kPB << 32) / range));
+                auto pb = [&](uint32_t x) { const uint32_t dx = x - plo; return __umulhi(dx, pmul) + (dx & pmask); };
+                for (uint32_t i = lane; i <= p0; i += kWave) {  // buckets (pb(P0[i-1]), pb(P0[i])] start at i
+                    const uint32_t bi = i < p0 ? pb(Cb[s + i]) : kPB;
+                    for (uint32_t b = i == 0 ? 0 : pb(Cb[s + i - 1]) + 1; b <= bi; b++) pbkt[b] = (uint16_t)i;
+                }
+                wave_sync();
+                uint32_t big = 0;
+                for (uint32_t b = lane; b < kPB; b += kWave) big = umax32(big, (uint32_t)pbkt[b + 1] - pbkt[b]);
+                big = (uint32_t)wave_max((uint64_t)big);
+                while (hb_b * 2 <= big) hb_b *= 2;
+            }
             for (uint64_t k0 = 0; k0 < d; k0 += kWave) {
                 const uint32_t nk = (uint32_t)umin64(kWave, d - k0);
                 uint32_t len = 0;
@@ -591,14 +617,29 @@ __global__ __launch_bounds__(256, 4) void mce_kernel(MceArgs a) {
 #pragma unroll
                     for (int q = 0; q < kBuildItems; q++)
                         w[q] = a.cols[fi[q] < T ? SL.start[own[q]] + (fi[q] - SL.pref[own[q]]) : lo];
+                    uint32_t lim[kBuildItems];
+                    uint32_t hb = hb_p;
 #pragma unroll
-                    for (int q = 0; q < kBuildItems; q++) pos[q] = (uint32_t)s;
-                    for (uint32_t step = hb_p; step >= 1; step >>= 1) {  // first P0 position with C >= w
+                    for (int q = 0; q < kBuildItems; q++) {
+                        pos[q] = (uint32_t)s;
+                        lim[q] = (uint32_t)d;
+                    }
+                    if (pbk) {  // (an entry outside P0's range lands in the last bucket and misses)
+                        hb = hb_b;
+#pragma unroll
+                        for (int q = 0; q < kBuildItems; q++) {
+                            const uint32_t dx = w[q] - plo;
+                            const uint32_t b = umin32(__umulhi(dx, pmul) + (dx & pmask), kPB - 1);
+                            pos[q] = (uint32_t)s + pbkt[b];
+                            lim[q] = (uint32_t)s + pbkt[b + 1];
+                        }
+                    }
+                    for (uint32_t step = hb; step >= 1; step >>= 1) {  // first P0 position with C >= w
 #pragma unroll
                         for (int q = 0; q < kBuildItems; q++) {
                             const uint32_t c = pos[q] + step;
                             const uint32_t cv = Cb[umin32(c, (uint32_t)d) - 1];
-                            pos[q] = (c <= d && cv < w[q]) ? c : pos[q];
+                            pos[q] = (c <= lim[q] && cv < w[q]) ? c : pos[q];
                         }
                     }
 #pragma unroll
