@@ -660,8 +660,8 @@ def main():
             out["learned_vs_plain_speedup"] = ex5["speedup_learned_vs_plain"]
             out["learned_vs_plain_note"] = (
                 f"maximal cliques on R-MAT scale {args.learned_scale} (cost model trained on 8 other seeds), "
-                "kernel time of the plain degeneracy order vs the learned order and splits, both with work "
-                "donation every 64 nodes; without donation: "
+                "kernel time of the plain degeneracy order vs the learned order (roots kept whole), both with "
+                "work donation every 64 nodes; without donation (learned order and first-level parts): "
                 f"{ex5['speedup_learned_vs_plain_nodonate']:.3f}x")
         out["extra"] = extra
     else:

@@ -154,7 +154,11 @@ RunResult Executor::run_bk(uint32_t hist_len, const BkForm& form) {
         const uint64_t waves = (uint64_t)cus * (form.tune.waves_per_cu ? form.tune.waves_per_cu : 16);
         order.resize(ns_);
         split.resize(ns_);
-        check(g2_learned_schedule(ctx_, mlp_, form.model->norm.mu, form.model->norm.sd, form.core, waves, 0,
+        // with work donation on, roots stay whole (order only): donation
+        // divides a heavy root where its work is, first-level parts each
+        // rebuild the root's bitsets
+        const uint32_t max_split = form.tune.mce_donate_every == 0xffffffffu ? 0u : 1u;
+        check(g2_learned_schedule(ctx_, mlp_, form.model->norm.mu, form.model->norm.sd, form.core, waves, max_split,
                                   order.data(), split.data(), nullptr),
               "learned schedule");
         for (uint32_t k : split) r.split_roots += k > 1;
