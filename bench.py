@@ -292,6 +292,7 @@ def run_social(args, g2, dev):
                             "wave_idle_frac": lr["wave_idle_frac"], "donations": lr["donations"],
                             "split_roots": lr["split_roots"], "consumer_kernel_ms": lr["consumer_kernel_ms"]},
         "speedup_vs_plain": pl["run_seconds"] / lr["run_seconds"],
+        "kernel_speedup_vs_plain": pl["kernel_ms"] / lr["kernel_ms"],
         "note": "run_seconds = wall time during which a consumer was running a view (features, inference and "
                 "the schedule included; materialisation and uploads, the same in both forms, excluded); the "
                 "four consumers share the GPU",

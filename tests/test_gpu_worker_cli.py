@@ -65,6 +65,21 @@ def test_bk_matches_oracle(orc, binned_graph):
     assert f"Total counts: {ref['total']}" in log
 
 
+def test_bk_reordered_and_split_bins(orc, binned_graph):
+    """-dr 1 orders the bins by their load factor and -ds 1 splits the bins
+    furthest above the mean until there are (4/2 + 1) x 2 = 6
+    (view/view_bin_manager.h's Reorder and Split): the sources are still
+    partitioned, so the total and histogram stay the oracle's, over 6 views."""
+    g, path, d = binned_graph
+    r, log = run_cli("bk", ["-dg", path, "-h", "2", "-pn", "2", "-cn", "2", "-t", "2", "-qs", "2",
+                            "-ss", "1", "-dr", "1", "-ds", "1"], d / "bk_split.json")
+    ref = orc.mce(g, hist_len=128)
+    assert r["total"] == ref["total"] > 0
+    assert {int(k): v for k, v in r["hist"].items()} == \
+        {int(k): int(ref["hist"][k]) for k in np.nonzero(ref["hist"])[0]}
+    assert r["plain"]["launches"] == 6
+
+
 @pytest.mark.parametrize("q", [0, 24])
 def test_run_matches_oracle(g2, orc, binned_graph, q):
     g, path, d = binned_graph
