@@ -75,6 +75,7 @@ constexpr int kB3Threads = 1024, kB3RowWords = 17664, kB3U = 1040, kB3Buckets = 
 constexpr uint32_t kRev = 0x80000000u;  // work-list flag: look the later members up in the row
 constexpr int kItems = 8;               // work items per thread in flight in the workgroup build
 constexpr int kItemsW = 2;              // work items per lane in flight in the wave build
+constexpr uint32_t kOwnScan = 8;        // a round or slice with at most this many owners after its first scans them
 constexpr int kRowUnroll = 8;           // row entries per lane in flight while building the oriented rows
 constexpr uint32_t kFetchRun = 8;       // roots per counter fetch in the first pass (a sweep of 4..64 favoured 8)
 constexpr uint32_t kWB = 256;           // value buckets over U for a root in the wave's LDS (|U| <= 149: u8 indices)
@@ -238,7 +239,15 @@ __device__ __forceinline__ void build_chunk(const LocalArgs& a, const uint32_t* 
         const uint32_t o1 = __popcll(ballot(lane < na && pref <= flast)) - 1;
 #pragma unroll
         for (int q = 0; q < kItemsW; q++) j[q] = o0;
-        if (o1 > o0) {
+        if (o1 > o0 && o1 - o0 <= kOwnScan) {
+            // few owners: count the member starts at or below each item, the
+            // starts read once per round as scalars
+            for (uint32_t o = o0 + 1; o <= o1; o++) {
+                const uint32_t po = read_lane(pref, o);
+#pragma unroll
+                for (int q = 0; q < kItemsW; q++) j[q] += po <= f[q] ? 1u : 0u;
+            }
+        } else if (o1 > o0) {
             uint32_t hb_o = 1;
             while (hb_o * 2 <= o1 - o0) hb_o *= 2;
             for (uint32_t step = hb_o; step >= 1; step >>= 1) {
@@ -717,7 +726,15 @@ __global__ __launch_bounds__(BT, 4) void gm_local_block_kernel(LocalArgs a) {
                 f[q] = f0 + q * kWave + lane;
                 own[q] = o0;
             }
-            if (o1 > o0) {
+            if (o1 > o0 && o1 - o0 <= kOwnScan) {
+                // few owners: count the member starts at or below each item,
+                // each start read once per slice as a scalar
+                for (uint32_t o = o0 + 1; o <= o1; o++) {
+                    const uint32_t po = __builtin_amdgcn_readfirstlane(mpref[o]);
+#pragma unroll
+                    for (int q = 0; q < kItems; q++) own[q] += po <= f[q] ? 1u : 0u;
+                }
+            } else if (o1 > o0) {
                 for (uint32_t step = hb_r; step >= 1; step >>= 1) {
 #pragma unroll
                     for (int q = 0; q < kItems; q++) {
