@@ -530,7 +530,11 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
 
     // roots come in runs of kFetchRun from the counter in the first pass (most
     // of them are trivial, and one atomic per root would queue the waves on
-    // the counter), singly in the third (few, heavy roots)
+    // the counter), singly in the third (few, heavy roots).  The first pass
+    // takes its roots from the top rank down: the roots with the largest
+    // local graphs sit in the upper middle of the (degree, id) order and the
+    // low ranks are the cheapest, so the pass ends on short roots instead of
+    // on a tail of long ones (14.5 -> 13.7 ms on R-MAT 22, 4-cliques).
     const uint32_t run = a.big_list ? kFetchRun : 1;
     uint64_t t = 0, tend = 0;
     while (true) {
@@ -539,7 +543,8 @@ __global__ __launch_bounds__(256, (N <= 4 ? 8 : 4)) void gm_local_kernel(LocalAr
             tend = umin64(t + run, a.ntasks);
         }
         if (t >= a.ntasks) break;
-        const uint32_t x = uniform(a.list ? a.list[t] : (uint32_t)t);
+        const uint64_t tt = a.big_list ? a.ntasks - 1 - t : t;
+        const uint32_t x = uniform(a.list ? a.list[tt] : (uint32_t)tt);
         t++;
         const uint64_t lo = uniform(a.orp[x]);
         const uint32_t m = (uint32_t)(uniform(a.orp[x + 1]) - lo);
