@@ -171,6 +171,14 @@ RunResult Executor::run_bk(uint32_t hist_len, const BkForm& form) {
           << (t_sched - st.t0) / 1e3 << " + device schedule " << (st.t1 - t_sched) / 1e3 << " ms";
         say(o.str());
     }
+    if (form.reverse && ns_) {
+        // the sources backwards: in the degeneracy order the roots with the
+        // largest neighbourhoods come last and leave a tail of long roots;
+        // taken first, the short roots fill in behind them
+        order.resize(ns_);
+        for (uint64_t i = 0; i < ns_; i++) order[i] = (uint32_t)(ns_ - 1 - i);
+        sched.order = order.data();
+    }
     auto account = [&](const g2_stats& st, uint64_t cl) {
         r.count += cl;
         for (uint32_t k = 0; k < hist_len; k++) r.hist[k] += h[k];
@@ -184,7 +192,7 @@ RunResult Executor::run_bk(uint32_t hist_len, const BkForm& form) {
     };
     g2_stats st{};
     uint64_t cl = 0;
-    int rc = g2_mce_run(ctx_, form.learned ? &sched : nullptr, 0, &cl, h.data(), hist_len, nullptr, &st);
+    int rc = g2_mce_run(ctx_, form.learned || form.reverse ? &sched : nullptr, 0, &cl, h.data(), hist_len, nullptr, &st);
     if (rc == G2_OK) {
         account(st, cl);
     } else if (rc == G2_EBUDGET && ns_ > 1) {
@@ -609,9 +617,10 @@ int run_worker(int argc, char** argv, App app) {
     if (gpus < 1) die("no GPU visible");
     if (pn < 1 || cn < 1) die("producer and consumer counts must be >= 1");
     if (qs < cn) die("queue size must be >= consumer count");
-    if (sched != "plain" && sched != "learned" && sched != "both") die("-sched must be plain, learned or both");
+    if (sched != "plain" && sched != "learned" && sched != "both" && sched != "reverse")
+        die("-sched must be plain, learned, both or reverse");
     if (app == App::GM && sched != "plain") die("-sched applies to bk only");
-    const bool want_learned = sched != "plain";
+    const bool want_learned = sched == "learned" || sched == "both";
     if (want_learned && model_in.empty() && train_out.empty()) die("-sched " + sched + " needs -model or -train");
     if (!train_out.empty() && train_gen.empty()) die("-train needs -train-gen <spec>[,<spec>...]");
     if (nb < 1) die("-nb must be >= 1");
@@ -728,7 +737,7 @@ int run_worker(int argc, char** argv, App app) {
     }
     std::vector<uint32_t> core;
     std::vector<std::pair<std::string, PipelineResult>> forms;
-    if (sched != "learned") {
+    if (sched == "plain" || sched == "both") {
         // the reference's schedule: source order, one root per wave, no donation
         pipe.form = BkForm{};
         if (app == App::BK) {
@@ -737,6 +746,15 @@ int run_worker(int argc, char** argv, App app) {
         }
         if (want_learned) say("== plain schedule ==");
         forms.push_back({"plain", pipe.run()});
+    }
+    if (sched == "reverse") {
+        // the reference's schedule with the sources backwards: one root per
+        // wave, no donation, no cost model
+        pipe.form = BkForm{};
+        pipe.form.reverse = true;
+        pipe.form.tune.mce_donate_every = 0xffffffffu;
+        pipe.form.tune.mce_roots_per_wave = 1;
+        forms.push_back({"reverse", pipe.run()});
     }
     if (want_learned) {
         say("== learned schedule ==");

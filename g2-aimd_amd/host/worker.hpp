@@ -202,6 +202,7 @@ struct Model {
 // how a consumer schedules the roots of its views
 struct BkForm {
     bool learned = false;
+    bool reverse = false;               // sources in reverse order, no cost model
     const Model* model = nullptr;       // learned form only
     const uint32_t* core = nullptr;     // core numbers of the whole graph (learned form)
     uint64_t nv = 0;

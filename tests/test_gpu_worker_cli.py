@@ -80,6 +80,21 @@ def test_bk_reordered_and_split_bins(orc, binned_graph):
     assert r["plain"]["launches"] == 6
 
 
+def test_bk_reverse_schedule(orc, binned_graph):
+    """-sched reverse takes each view's sources backwards (the roots with the
+    largest neighbourhoods first) through g2_sched.order, with no cost model:
+    the order of the roots changes, the total and histogram do not."""
+    g, path, d = binned_graph
+    r, log = run_cli("bk", ["-dg", path, "-h", "2", "-pn", "2", "-cn", "2", "-t", "2", "-qs", "2",
+                            "-ss", "1", "-sched", "reverse"], d / "bk_reverse.json")
+    ref = orc.mce(g, hist_len=128)
+    assert r["total"] == ref["total"] > 0
+    assert {int(k): v for k, v in r["hist"].items()} == \
+        {int(k): int(ref["hist"][k]) for k in np.nonzero(ref["hist"])[0]}
+    assert "reverse" in r and "plain" not in r
+    assert r["reverse"]["launches"] == 4
+
+
 @pytest.mark.parametrize("q", [0, 24])
 def test_run_matches_oracle(g2, orc, binned_graph, q):
     g, path, d = binned_graph
